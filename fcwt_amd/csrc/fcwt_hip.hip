@@ -1,0 +1,787 @@
+// fcwt_hip.hip -- gfx950 implementation of fCWT's FCWT::cwt hot path behind
+// the C ABI declared in include/fcwt_hip.h.
+//
+// The reference (src/fcwt/fcwt.cpp:447-529) pads the real signal to
+// N = 2^find2power(n), takes one r2c FFT, then for every scale multiplies the
+// one-sided spectrum by the Morlet window sampled from one mother table and
+// runs a full N-point inverse FFT, keeping the first n samples.
+//
+// Here each scale's inverse DFT is band-limited: its spectrum is zero from
+// bin `band` up (band <= N/2), so with M = 2^ceil(log2 band) the row factors
+// as N = B * F with B <= kMaxB:
+//   y[f + F*m] = sum_{qb<B} W_B^{qb*m} * U[qb][f],
+//   U[qb][f]   = W_N^{qb*f} * sum_{qf<A} Z[qb + B*qf] W_F^{qf*f},  A = M/B
+// where Z[q] = X[q] * mother[idx(q)] and W_K = e^{2 pi i / K}.
+//   * "direct" rows (A == 1): the writer kernel forms U on the fly from the
+//     spectrum and does G batched B-point inverse FFTs in LDS for G
+//     consecutive f, writing G-contiguous runs of the output row.
+//   * "two-pass" rows (A > 1): a spreader kernel does the F-point inverse
+//     FFTs over qf and writes U to a scratch buffer ([f][qb] layout); the
+//     writer then reads U instead of forming it.
+// The forward FFT is a half-length complex FFT of the packed signal plus the
+// usual real post-pass (one work-group for N/2 <= kLdsCap, otherwise two
+// four-step passes).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "fcwt_hip.h"
+#include "fft_lds.h"
+#include "host_math.h"
+
+namespace fcwt_hip {
+
+constexpr int kMaxLog2B = 10;  // writer FFT size cap: G * B <= kLdsCap with G >= 16
+constexpr double kStaleFloor = 1e-12;
+constexpr size_t kScratchBytes = size_t(256) << 20;
+
+struct SegD {
+  int lo, hi;
+  float step;
+  int pad;
+};
+
+struct RowD {
+  int seg_begin, seg_count;
+  int log2B;   // writer FFT size
+  int log2A;   // spreader band factor (0: direct row)
+  int log2G;   // consecutive f per writer block
+  int slot;    // scratch slot (two-pass rows)
+  long long out_off;
+};
+
+struct Item {
+  int row;
+  int block;
+};
+
+struct KArgs {
+  const float2* spec;   // X[0, N/2)
+  const float* mother;  // N values
+  const SegD* segs;
+  const RowD* rows;
+  const Item* items;
+  float2* scratch;
+  float2* out;
+  int n;
+  int N;
+  int log2N;
+  float norm;
+  Twiddles tw;
+};
+
+// Z[q] = X[q] * mother[(int)fminf(N-1, step*q)] for the segment holding q
+// (fcwt.cpp:224-248), 0 outside every segment.
+__device__ __forceinline__ float2 spectrum_bin(const KArgs& a, const RowD& r, int q) {
+#pragma clang fp contract(off)
+  for (int s = 0; s < r.seg_count; ++s) {
+    SegD sg = a.segs[r.seg_begin + s];
+    if (q >= sg.lo && q < sg.hi) {
+      float v = sg.step * (float)q;
+      float mx = (float)a.N - 1.0f;
+      int idx = (int)(v < mx ? v : mx);
+      float w = a.mother[idx];
+      float2 x = a.spec[q];
+      return make_float2(x.x * w, x.y * w);
+    }
+  }
+  return make_float2(0.f, 0.f);
+}
+
+// Writer: one work-group per (row, block of G consecutive f).
+__global__ __launch_bounds__(kBlock) void k_writer(KArgs a) {
+  __shared__ float2 lds[kLdsCap];
+  const Item it = a.items[blockIdx.x];
+  const RowD r = a.rows[it.row];
+  const int log2B = r.log2B, B = 1 << log2B;
+  const int log2G = r.log2G, G = 1 << log2G;
+  const int log2F = a.log2N - log2B;
+  const int F = 1 << log2F;
+  const int f0 = it.block << log2G;
+  const int tot = B << log2G;
+  if (r.log2A == 0) {
+    for (int e = threadIdx.x; e < tot; e += kBlock) {
+      int g = e & (G - 1), qb = e >> log2G;
+      float2 z = spectrum_bin(a, r, qb);
+      float2 w = a.tw((unsigned)(qb * (f0 + g)));
+      lds[e] = cmul(z, w);
+    }
+  } else {
+    const float2* U = a.scratch + (size_t)r.slot * a.N;
+    for (int e = threadIdx.x; e < tot; e += kBlock) {
+      int qb = e & (B - 1), g = e >> log2B;
+      lds[(qb << log2G) + g] = U[(size_t)(f0 + g) * B + qb];
+    }
+  }
+  __syncthreads();
+  lds_fft(lds, log2B, G, log2G, a.tw, 1.0f);
+  float2* out = a.out + r.out_off;
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int g = e & (G - 1), m = e >> log2G;
+    long long t = (long long)(f0 + g) + (long long)F * m;
+    if (t < a.n) out[t] = cscale(lds[e], a.norm);
+  }
+}
+
+// Spreader: one work-group per (two-pass row, block of Q consecutive qb);
+// F-point inverse FFT over qf for each qb, twiddle by W_N^{qb*f}, store
+// U[f][qb] in the row's scratch slot.
+__global__ __launch_bounds__(kBlock) void k_spreader(KArgs a) {
+  __shared__ float2 lds[kLdsCap];
+  const Item it = a.items[blockIdx.x];
+  const RowD r = a.rows[it.row];
+  const int log2B = r.log2B, B = 1 << log2B;
+  const int log2F = a.log2N - log2B;
+  const int F = 1 << log2F;
+  const int A = 1 << r.log2A;
+  const int log2Q = min(log2B, 14 - log2F);
+  const int Q = 1 << log2Q;
+  const int qb0 = it.block << log2Q;
+  const int tot = F << log2Q;
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int j = e & (Q - 1), qf = e >> log2Q;
+    float2 z = make_float2(0.f, 0.f);
+    if (qf < A) z = spectrum_bin(a, r, qb0 + j + (qf << log2B));
+    lds[e] = z;
+  }
+  __syncthreads();
+  lds_fft(lds, log2F, Q, log2Q, a.tw, 1.0f);
+  float2* U = a.scratch + (size_t)r.slot * a.N;
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int j = e & (Q - 1), f = e >> log2Q;
+    int qb = qb0 + j;
+    U[(size_t)f * B + qb] = cmul(lds[e], a.tw((unsigned)(qb * f)));
+  }
+}
+
+struct FwdArgs {
+  const float* x;
+  int n;
+  float2* spec;  // X[0, H)
+  float2* ztmp;  // H (four-step only)
+  float2* atmp;  // H (four-step only)
+  int log2H;
+  int log2H1, log2H2;
+  Twiddles tw;
+};
+
+// packed half-length sample z[j] = x[2j] + i x[2j+1] (zero past n)
+__device__ __forceinline__ float2 packed(const FwdArgs& a, int j) {
+  long long i0 = 2LL * j;
+  float re = i0 < a.n ? a.x[i0] : 0.f;
+  float im = i0 + 1 < a.n ? a.x[i0 + 1] : 0.f;
+  return make_float2(re, im);
+}
+
+// X[k] = (Z[k] + conj Z[H-k]) / 2 - (i/2) e^{-2 pi i k/N} (Z[k] - conj Z[H-k])
+__device__ __forceinline__ float2 real_post(float2 zk, float2 zc, float2 w) {
+  float2 e = make_float2(0.5f * (zk.x + zc.x), 0.5f * (zk.y - zc.y));
+  float2 o = make_float2(0.5f * (zk.y + zc.y), -0.5f * (zk.x - zc.x));  // (Z[k] - conj Z[H-k]) / (2i)
+  return cadd(e, cmul(w, o));
+}
+
+// Whole forward transform in one work-group (H = N/2 <= kLdsCap).
+__global__ __launch_bounds__(kBlock) void k_fwd_small(FwdArgs a) {
+  __shared__ float2 lds[kLdsCap];
+  const int H = 1 << a.log2H;
+  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(a, j);
+  __syncthreads();
+  lds_fft(lds, a.log2H, 1, 0, a.tw, -1.0f);
+  for (int k = threadIdx.x; k < H; k += kBlock) {
+    float2 zk = lds[k], zc = lds[(H - k) & (H - 1)];
+    a.spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
+  }
+}
+
+// Four-step pass 1: H1-point FFTs over j1 of z[j2 + H2*j1] for G consecutive
+// j2, times W_H^{-j2*k1}, stored to atmp[k1*H2 + j2].
+__global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
+  __shared__ float2 lds[kLdsCap];
+  const int H1 = 1 << a.log2H1, H2 = 1 << a.log2H2;
+  const int log2G = min(a.log2H2, 14 - a.log2H1);
+  const int G = 1 << log2G;
+  const int j20 = blockIdx.x << log2G;
+  const int tot = H1 << log2G;
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int g = e & (G - 1), j1 = e >> log2G;
+    lds[e] = packed(a, j20 + g + H2 * j1);
+  }
+  __syncthreads();
+  lds_fft(lds, a.log2H1, G, log2G, a.tw, -1.0f);
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int g = e & (G - 1), k1 = e >> log2G;
+    int j2 = j20 + g;
+    a.atmp[(size_t)k1 * H2 + j2] = cmul(lds[e], a.tw.get((unsigned)(2 * j2 * k1), -1.0f));
+  }
+}
+
+// Four-step pass 2: H2-point FFTs over j2 of atmp[k1*H2 + j2] for G
+// consecutive k1, stored to ztmp[k1 + H1*k2].
+__global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
+  __shared__ float2 lds[kLdsCap];
+  const int H1 = 1 << a.log2H1, H2 = 1 << a.log2H2;
+  const int log2G = min(a.log2H1, 14 - a.log2H2);
+  const int G = 1 << log2G;
+  const int k10 = blockIdx.x << log2G;
+  const int tot = H2 << log2G;
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int j2 = e & (H2 - 1), g = e >> a.log2H2;
+    lds[(j2 << log2G) + g] = a.atmp[(size_t)(k10 + g) * H2 + j2];
+  }
+  __syncthreads();
+  lds_fft(lds, a.log2H2, G, log2G, a.tw, -1.0f);
+  for (int e = threadIdx.x; e < tot; e += kBlock) {
+    int g = e & (G - 1), k2 = e >> log2G;
+    a.ztmp[(size_t)(k10 + g) + (size_t)H1 * k2] = lds[e];
+  }
+}
+
+__global__ void k_fwd_post(FwdArgs a) {
+  const int H = 1 << a.log2H;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < H) a.spec[k] = real_post(a.ztmp[k], a.ztmp[(H - k) & (H - 1)], a.tw.get((unsigned)k, -1.0f));
+}
+
+// Mother table (fcwt.cpp:51-67): the argument in float32 exactly as the
+// reference forms it, the exponential in float64 rounded to float32.
+__global__ void k_mother(float* mother, int N, float fb, float toradians, float norm) {
+#pragma clang fp contract(off)
+  const float kPi = 3.14159265358979323846264338327950288f;
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w < N) {
+    float a = (2.0f * ((float)w * toradians) * fb - 2.0f * kPi * fb);
+    a = -(a * a) / 2;
+    mother[w] = norm * (float)exp((double)a);
+  }
+}
+
+// Twiddle tables, float64 sincospi rounded to float32.
+__global__ void k_twiddles(float2* t1, int n1, float2* t2, int n2, int s1bits, int log2N) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double invN = ldexp(1.0, -log2N);
+  if (i < n1) {
+    double s, c;
+    sincospi(2.0 * i * invN, &s, &c);
+    t1[i] = make_float2((float)c, (float)s);
+  }
+  if (i < n2) {
+    double s, c;
+    sincospi(2.0 * ldexp((double)i, s1bits) * invN, &s, &c);
+    t2[i] = make_float2((float)c, (float)s);
+  }
+}
+
+}  // namespace fcwt_hip
+
+using namespace fcwt_hip;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(FCWT_HIP_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// -------------------------------------------------------------------- plan
+struct fcwt_hip_plan_s {
+  int device = 0;
+  int n = 0, N = 1, log2N = 0, fn = 0, threads = 1;
+  float fb = 0.f;
+  bool normalize = true;
+  unsigned flags = 0;
+  std::vector<float> scales;
+  // schedule
+  std::vector<RowD> rows;
+  std::vector<SegD> segs;
+  std::vector<Item> items;  // direct writer items, then per batch: spreader items, writer items
+  size_t direct_items = 0;
+  struct Batch {
+    size_t s_begin, s_count, w_begin, w_count;
+  };
+  std::vector<Batch> batches;
+  int scratch_slots = 0;
+  // device
+  float* d_mother = nullptr;
+  float2* d_tw1 = nullptr;
+  float2* d_tw2 = nullptr;
+  int s1bits = 0;
+  float2* d_spec = nullptr;
+  float2* d_ztmp = nullptr;
+  float2* d_atmp = nullptr;
+  SegD* d_segs = nullptr;
+  RowD* d_rows = nullptr;
+  Item* d_items = nullptr;
+  float2* d_scratch = nullptr;
+  float* d_x = nullptr;
+  float2* d_out = nullptr;
+  hipStream_t stream = nullptr;
+  int log2H1 = 0, log2H2 = 0;
+  size_t bytes = 0;
+  // per-kernel timing (fcwt_hip_plan_profile): event pairs around launches
+  struct Timed {
+    int kind;
+    hipEvent_t a, b;
+  };
+  bool profiling = false;
+  std::vector<Timed> pending;
+  std::vector<hipEvent_t> event_pool;
+  double prof_ms[4] = {0, 0, 0, 0};
+  long long prof_launches[4] = {0, 0, 0, 0};
+};
+
+// kernel kinds reported by fcwt_hip_plan_profile_read
+static const char* const kKindNames[4] = {"forward_fft", "row_writer_direct", "row_spreader", "row_writer_two_pass"};
+
+static hipEvent_t take_event(fcwt_hip_plan p) {
+  if (!p->event_pool.empty()) {
+    hipEvent_t e = p->event_pool.back();
+    p->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Records an event before (begin) / after (end) a launch of `kind` on st.
+struct ProfScope {
+  fcwt_hip_plan p;
+  int kind;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  ProfScope(fcwt_hip_plan p_, int k, hipStream_t s) : p(p_), kind(k), st(s) {
+    if (p->profiling && (a = take_event(p))) (void)hipEventRecord(a, st);
+  }
+  ~ProfScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(p);
+    if (b && hipEventRecord(b, st) == hipSuccess)
+      p->pending.push_back({kind, a, b});
+    else
+      p->event_pool.push_back(a);
+  }
+};
+
+static void drain_profile(fcwt_hip_plan p) {
+  for (auto& t : p->pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(t.b) == hipSuccess && hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) {
+      p->prof_ms[t.kind] += ms;
+      p->prof_launches[t.kind] += 1;
+    }
+    p->event_pool.push_back(t.a);
+    p->event_pool.push_back(t.b);
+  }
+  p->pending.clear();
+}
+
+static void free_plan(fcwt_hip_plan p) {
+  if (!p) return;
+  int cur = 0;
+  (void)hipGetDevice(&cur);
+  (void)hipSetDevice(p->device);
+  drain_profile(p);
+  for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
+  void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
+                  p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  (void)hipSetDevice(cur);
+  delete p;
+}
+
+template <class T>
+static int dalloc(fcwt_hip_plan p, T** ptr, size_t count) {
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
+  if (e != hipSuccess) {
+    *ptr = nullptr;
+    return fail(FCWT_HIP_ENOMEM, std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) +
+                                     " bytes failed: " + hipGetErrorString(e));
+  }
+  p->bytes += count * sizeof(T);
+  return FCWT_HIP_OK;
+}
+
+static Twiddles twiddles_of(const fcwt_hip_plan p) {
+  Twiddles t;
+  t.t1 = p->d_tw1;
+  t.t2 = p->d_tw2;
+  t.s1bits = p->s1bits;
+  t.log2N = p->log2N;
+  return t;
+}
+
+static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
+  const bool avx = !(p->flags & FCWT_HIP_COVERAGE_SCALAR);
+  std::vector<RowBand> bands =
+      build_bands(p->N, p->scales.data(), p->fn, p->threads, avx, mother_host.data(), kStaleFloor);
+  p->rows.resize(p->fn);
+  std::vector<int> wide;
+  for (int i = 0; i < p->fn; ++i) {
+    RowD& r = p->rows[i];
+    r.seg_begin = (int)p->segs.size();
+    r.seg_count = (int)bands[i].segs.size();
+    for (const Segment& s : bands[i].segs) p->segs.push_back({s.lo, s.hi, s.step, 0});
+    int log2M = find2power(std::max(bands[i].band, 1));
+    log2M = std::min(log2M, p->log2N);
+    r.log2B = std::min(log2M, kMaxLog2B);
+    r.log2A = log2M - r.log2B;
+    int log2F = p->log2N - r.log2B;
+    r.log2G = std::min(log2F, 14 - r.log2B);
+    r.slot = -1;
+    r.out_off = (long long)i * p->n;
+    if (r.log2A > 0) wide.push_back(i);
+  }
+  // direct writer items
+  for (int i = 0; i < p->fn; ++i) {
+    const RowD& r = p->rows[i];
+    if (r.log2A != 0) continue;
+    int nblk = 1 << (p->log2N - r.log2B - r.log2G);
+    for (int b = 0; b < nblk; ++b) p->items.push_back({i, b});
+  }
+  p->direct_items = p->items.size();
+  if (!wide.empty()) {
+    if (p->log2N - kMaxLog2B > 14)
+      return fail(FCWT_HIP_EUNSUPPORTED, "padded size N = 2^" + std::to_string(p->log2N) +
+                                             " exceeds this build's limit of 2^24 for wide scales");
+    size_t per_row = sizeof(float2) * (size_t)p->N;
+    p->scratch_slots = (int)std::max<size_t>(1, std::min<size_t>(wide.size(), kScratchBytes / per_row));
+    for (size_t b0 = 0; b0 < wide.size(); b0 += p->scratch_slots) {
+      size_t b1 = std::min(wide.size(), b0 + p->scratch_slots);
+      fcwt_hip_plan_s::Batch bt;
+      bt.s_begin = p->items.size();
+      for (size_t k = b0; k < b1; ++k) {
+        RowD& r = p->rows[wide[k]];
+        r.slot = (int)(k - b0);
+        int log2F = p->log2N - r.log2B;
+        int log2Q = std::min(r.log2B, 14 - log2F);
+        int nblk = 1 << (r.log2B - log2Q);
+        for (int b = 0; b < nblk; ++b) p->items.push_back({wide[k], b});
+      }
+      bt.s_count = p->items.size() - bt.s_begin;
+      bt.w_begin = p->items.size();
+      for (size_t k = b0; k < b1; ++k) {
+        const RowD& r = p->rows[wide[k]];
+        int nblk = 1 << (p->log2N - r.log2B - r.log2G);
+        for (int b = 0; b < nblk; ++b) p->items.push_back({wide[k], b});
+      }
+      bt.w_count = p->items.size() - bt.w_begin;
+      p->batches.push_back(bt);
+    }
+  }
+  return FCWT_HIP_OK;
+}
+
+static int check_device(int* dev) {
+  HIP_TRY(hipGetDevice(dev));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, *dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(FCWT_HIP_EDEVICE, std::string("libfcwt_hip is built for gfx950; current device is ") + prop.gcnArchName);
+  return FCWT_HIP_OK;
+}
+
+#define TRY(expr)                 \
+  do {                            \
+    int rc_ = (expr);             \
+    if (rc_ != FCWT_HIP_OK) {     \
+      free_plan(p);               \
+      return rc_;                 \
+    }                             \
+  } while (0)
+
+static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStream_t st) {
+  // forward transform
+  FwdArgs f;
+  f.x = d_x;
+  f.n = p->n;
+  f.spec = p->d_spec;
+  f.ztmp = p->d_ztmp;
+  f.atmp = p->d_atmp;
+  f.log2H = p->log2N - 1;
+  f.log2H1 = p->log2H1;
+  f.log2H2 = p->log2H2;
+  f.tw = twiddles_of(p);
+  if (p->N >= 2) {
+    ProfScope ps(p, 0, st);
+    if (f.log2H <= 14) {
+      k_fwd_small<<<1, kBlock, 0, st>>>(f);
+    } else {
+      int ga = 1 << (p->log2H2 - std::min(p->log2H2, 14 - p->log2H1));
+      int gb = 1 << (p->log2H1 - std::min(p->log2H1, 14 - p->log2H2));
+      k_fwd_a<<<ga, kBlock, 0, st>>>(f);
+      k_fwd_b<<<gb, kBlock, 0, st>>>(f);
+      int H = 1 << f.log2H;
+      k_fwd_post<<<(H + 255) / 256, 256, 0, st>>>(f);
+    }
+  }
+  KArgs a;
+  a.spec = p->d_spec;
+  a.mother = p->d_mother;
+  a.segs = p->d_segs;
+  a.rows = p->d_rows;
+  a.scratch = p->d_scratch;
+  a.out = d_out;
+  a.n = p->n;
+  a.N = p->N;
+  a.log2N = p->log2N;
+  a.norm = p->normalize ? 1.0f / (float)p->N : 1.0f;
+  a.tw = twiddles_of(p);
+  if (p->direct_items) {
+    ProfScope ps(p, 1, st);
+    a.items = p->d_items;
+    k_writer<<<(unsigned)p->direct_items, kBlock, 0, st>>>(a);
+  }
+  for (const auto& bt : p->batches) {
+    {
+      ProfScope ps(p, 2, st);
+      a.items = p->d_items + bt.s_begin;
+      k_spreader<<<(unsigned)bt.s_count, kBlock, 0, st>>>(a);
+    }
+    ProfScope ps(p, 3, st);
+    a.items = p->d_items + bt.w_begin;
+    k_writer<<<(unsigned)bt.w_count, kBlock, 0, st>>>(a);
+  }
+  HIP_TRY(hipGetLastError());
+  return FCWT_HIP_OK;
+}
+
+// ------------------------------------------------------------------- C ABI
+extern "C" {
+
+const char* fcwt_hip_last_error(void) { return g_last_error.c_str(); }
+
+const char* fcwt_hip_version(void) { return "fcwt_hip 0.1.0 (gfx950)"; }
+
+int fcwt_hip_find2power(int n) { return find2power(n); }
+
+int fcwt_hip_scales(int scaletype, int fs, float f0, float f1, int fn, float* out) {
+  if (!out || fn < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_scales: need fn >= 1 and an output buffer");
+  if (scaletype < 0 || scaletype > 2) return fail(FCWT_HIP_EINVAL, "fcwt_hip_scales: unknown scale type");
+  scales(scaletype, fs, f0, f1, fn, out);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_frequencies(const float* sc, int fn, int fs, float* out) {
+  if (!sc || !out || fn < 0) return fail(FCWT_HIP_EINVAL, "fcwt_hip_frequencies: bad arguments");
+  frequencies(sc, fn, fs, out);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_morlet_generate(float fb, int size, float* mother) {
+  if (!mother || size < 0) return fail(FCWT_HIP_EINVAL, "fcwt_hip_morlet_generate: bad arguments");
+  morlet_mother(fb, size, mother);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_morlet_support(float fb, float scale) { return morlet_support(fb, scale); }
+
+int fcwt_hip_morlet_generate_time(float fb, float scale, int size, float* real, float* imag) {
+  if (!real || !imag) return fail(FCWT_HIP_EINVAL, "fcwt_hip_morlet_generate_time: bad arguments");
+  morlet_time(fb, scale, size, real, imag);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_morlet_wavelet(float fb, float scale, float* out, int pn) {
+  if (!out || pn < 0) return fail(FCWT_HIP_EINVAL, "fcwt_hip_morlet_wavelet: bad arguments");
+  morlet_wavelet(fb, scale, out, pn);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_describe_rows(int n, const float* sc, int fn, float fb, int threads, unsigned flags, int* covered,
+                           int* band, int* segments) {
+  if (n < 1 || fn < 1 || !sc || threads < 1 || find2power(n) > 28)
+    return fail(FCWT_HIP_EINVAL, "fcwt_hip_describe_rows: bad arguments");
+  int N = 1 << find2power(n);
+  std::vector<float> mother(N);
+  morlet_mother(fb, N, mother.data());
+  std::vector<RowBand> b = build_bands(N, sc, fn, threads, !(flags & FCWT_HIP_COVERAGE_SCALAR), mother.data(),
+                                       kStaleFloor);
+  for (int i = 0; i < fn; ++i) {
+    if (covered) covered[i] = b[i].C;
+    if (band) band[i] = b[i].band;
+    if (segments) segments[i] = (int)b[i].segs.size();
+  }
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, float fb, int threads, int normalize,
+                         unsigned flags) {
+  if (!out) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: null plan pointer");
+  *out = nullptr;
+  if (n < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: signal length must be >= 1");
+  if (fn < 1 || !sc) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: need fn >= 1 scales");
+  if (threads < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: threads must be >= 1");
+  if (!std::isfinite(fb)) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: wavelet bandwidth must be finite");
+  for (int i = 0; i < fn; ++i)
+    if (!(sc[i] > 0.f) || !std::isfinite(sc[i]))
+      return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: scales must be finite and > 0");
+  if (flags & ~FCWT_HIP_COVERAGE_SCALAR) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: unknown flags");
+  int log2N = find2power(n);
+  if (log2N > 28) return fail(FCWT_HIP_EUNSUPPORTED, "fcwt_hip_plan_create: signal too long");
+  int dev = 0;
+  int rc = check_device(&dev);
+  if (rc != FCWT_HIP_OK) return rc;
+
+  fcwt_hip_plan p = new (std::nothrow) fcwt_hip_plan_s;
+  if (!p) return fail(FCWT_HIP_ENOMEM, "fcwt_hip_plan_create: out of host memory");
+  p->device = dev;
+  p->n = n;
+  p->log2N = log2N;
+  p->N = 1 << log2N;
+  p->fn = fn;
+  p->fb = fb;
+  p->threads = threads;
+  p->normalize = normalize != 0;
+  p->flags = flags;
+  p->scales.assign(sc, sc + fn);
+
+  // host copy of the mother table, used only to judge which stale bins matter
+  std::vector<float> mother_host(p->N);
+  morlet_mother(fb, p->N, mother_host.data());
+  TRY(build_schedule(p, mother_host));
+
+  if (hipStreamCreate(&p->stream) != hipSuccess) {
+    free_plan(p);
+    return fail(FCWT_HIP_EDEVICE, "hipStreamCreate failed");
+  }
+  const int N = p->N;
+  p->s1bits = (log2N + 1) / 2;
+  int n1 = 1 << p->s1bits, n2 = N >> p->s1bits;
+  TRY(dalloc(p, &p->d_mother, N));
+  TRY(dalloc(p, &p->d_tw1, n1));
+  TRY(dalloc(p, &p->d_tw2, n2));
+  TRY(dalloc(p, &p->d_spec, std::max(1, N / 2)));
+  int log2H = log2N - 1;
+  if (log2H > 14) {
+    p->log2H1 = log2H / 2;
+    p->log2H2 = log2H - p->log2H1;
+    TRY(dalloc(p, &p->d_ztmp, (size_t)1 << log2H));
+    TRY(dalloc(p, &p->d_atmp, (size_t)1 << log2H));
+  }
+  TRY(dalloc(p, &p->d_segs, p->segs.size()));
+  TRY(dalloc(p, &p->d_rows, p->rows.size()));
+  TRY(dalloc(p, &p->d_items, p->items.size()));
+  if (p->scratch_slots) TRY(dalloc(p, &p->d_scratch, (size_t)p->scratch_slots * N));
+
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (bytes == 0) return FCWT_HIP_OK;
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, p->stream));
+    return FCWT_HIP_OK;
+  };
+  TRY(up(p->d_segs, p->segs.data(), p->segs.size() * sizeof(SegD)));
+  TRY(up(p->d_rows, p->rows.data(), p->rows.size() * sizeof(RowD)));
+  TRY(up(p->d_items, p->items.data(), p->items.size() * sizeof(Item)));
+
+  // mother-table scalars in float32 exactly as fcwt.cpp:56-57 forms them
+  const float kPi = 3.14159265358979323846264338327950288f;
+  float toradians = (2 * kPi) / (float)N;
+  float norm = sqrtf(2 * kPi) * 0.75112554446f;
+  k_mother<<<(N + 255) / 256, 256, 0, p->stream>>>(p->d_mother, N, fb, toradians, norm);
+  int nt = std::max(n1, n2);
+  k_twiddles<<<(nt + 255) / 256, 256, 0, p->stream>>>(p->d_tw1, n1, p->d_tw2, n2, p->s1bits, log2N);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) {
+    free_plan(p);
+    return fail(FCWT_HIP_EDEVICE, "fcwt_hip_plan_create: table kernels failed");
+  }
+  *out = p;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_destroy(fcwt_hip_plan p) {
+  free_plan(p);
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_shape(fcwt_hip_plan p, int* n, int* N, int* fn) {
+  if (!p) return fail(FCWT_HIP_EINVAL, "null plan");
+  if (n) *n = p->n;
+  if (N) *N = p->N;
+  if (fn) *fn = p->fn;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_profile(fcwt_hip_plan p, int enable) {
+  if (!p) return fail(FCWT_HIP_EINVAL, "null plan");
+  drain_profile(p);
+  p->profiling = enable != 0;
+  for (int k = 0; k < 4; ++k) {
+    p->prof_ms[k] = 0;
+    p->prof_launches[k] = 0;
+  }
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_profile_read(fcwt_hip_plan p, int max, const char** names, double* ms, long long* launches,
+                               int* count) {
+  if (!p || max < 0) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_profile_read: bad arguments");
+  drain_profile(p);
+  int k = 0;
+  for (int i = 0; i < 4; ++i) {
+    if (p->prof_launches[i] == 0) continue;
+    if (k < max) {
+      if (names) names[k] = kKindNames[i];
+      if (ms) ms[k] = p->prof_ms[i];
+      if (launches) launches[k] = p->prof_launches[i];
+    }
+    ++k;
+  }
+  if (count) *count = k;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_device_bytes(fcwt_hip_plan p, size_t* bytes) {
+  if (!p || !bytes) return fail(FCWT_HIP_EINVAL, "null argument");
+  *bytes = p->bytes;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_cwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, void* stream) {
+  if (!p || !d_input || !d_output) return fail(FCWT_HIP_EINVAL, "fcwt_hip_cwt_device: null argument");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  int rc = launch_all(p, d_input, (float2*)d_output, (hipStream_t)stream);
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
+}
+
+int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
+  if (!p || !input || !output) return fail(FCWT_HIP_EINVAL, "fcwt_hip_cwt: null argument");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  int rc = FCWT_HIP_OK;
+  const size_t out_elems = (size_t)p->fn * p->n;
+  if (!p->d_x) rc = dalloc(p, &p->d_x, p->n);
+  if (rc == FCWT_HIP_OK && !p->d_out) rc = dalloc(p, &p->d_out, out_elems);
+  if (rc == FCWT_HIP_OK) {
+    hipError_t e = hipMemcpyAsync(p->d_x, input, sizeof(float) * p->n, hipMemcpyHostToDevice, p->stream);
+    if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("input copy failed: ") + hipGetErrorString(e));
+  }
+  if (rc == FCWT_HIP_OK) rc = launch_all(p, p->d_x, p->d_out, p->stream);
+  if (rc == FCWT_HIP_OK) {
+    hipError_t e = hipMemcpyAsync(output, p->d_out, sizeof(float2) * out_elems, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("cwt failed: ") + hipGetErrorString(e));
+  }
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+// fft_lds.h -- batched power-of-two complex FFTs in LDS for gfx950, and the
+// twiddle source they share.  Device code, internal to libfcwt_hip.
+//
+// Layout: a batch of G transforms of P points each lives in LDS as
+// buf[e * G + g] (element e of transform g), so consecutive lanes touch
+// consecutive 8-byte words and every access is bank-conflict free.
+// The transform is Stockham autosort (natural order in and out), in place:
+// each stage reads its butterflies into registers, waits at a barrier, writes
+// them back to their sorted positions, waits again.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fcwt_hip {
+
+constexpr int kLdsCap = 16384;  // complex64 elements per work-group (128 KiB)
+constexpr int kBlock = 1024;    // threads per LDS-FFT work-group
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// multiply by s*i (s = +1 or -1)
+__device__ __forceinline__ float2 cmul_si(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
+
+// e^{+2 pi i r / N} for 0 <= r < N from two float64-generated tables:
+// t1[j] = e^{2 pi i j / N} (j < 2^s1bits), t2[k] = e^{2 pi i k 2^s1bits / N}.
+struct Twiddles {
+  const float2* t1;
+  const float2* t2;
+  int s1bits;
+  int log2N;
+  __device__ __forceinline__ float2 operator()(unsigned r) const {
+    float2 a = t2[r >> s1bits];
+    float2 b = t1[r & ((1u << s1bits) - 1u)];
+    return cmul(a, b);
+  }
+  // e^{sign * 2 pi i r / N}
+  __device__ __forceinline__ float2 get(unsigned r, float sign) const {
+    float2 w = (*this)(r);
+    w.y *= sign;
+    return w;
+  }
+};
+
+// One Stockham stage of radix R over G interleaved transforms of 2^log2P
+// points; Ns = 2^log2Ns is the size of the sub-transforms already done.
+template <int R>
+__device__ __forceinline__ void stockham_stage(float2* buf, int log2P, int log2Ns, int G, int log2G,
+                                               const Twiddles& tw, float sign) {
+  constexpr int log2R = R == 2 ? 1 : (R == 4 ? 2 : 3);
+  const int P = 1 << log2P;
+  const int nbf = (P >> log2R) << log2G;  // butterflies in the stage
+  constexpr int kMaxPer = kLdsCap / R / kBlock;
+  float2 v[kMaxPer][R];
+  const int Ns = 1 << log2Ns;
+  const int twshift = tw.log2N - (log2Ns + log2R);
+#pragma unroll
+  for (int u = 0; u < kMaxPer; ++u) {
+    int b = threadIdx.x + u * kBlock;
+    if (b < nbf) {
+      int g = b & (G - 1);
+      int j = b >> log2G;
+      int k = j & (Ns - 1);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[u][r] = buf[((j + r * (P >> log2R)) << log2G) + g];
+      if (log2Ns > 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[u][r] = cmul(v[u][r], tw.get((unsigned)(r * k) << twshift, sign));
+      }
+      if (R == 2) {
+        float2 a = v[u][0], c = v[u][1];
+        v[u][0] = cadd(a, c);
+        v[u][1] = csub(a, c);
+      } else if (R == 4) {
+        float2 a0 = cadd(v[u][0], v[u][2]), a1 = csub(v[u][0], v[u][2]);
+        float2 b0 = cadd(v[u][1], v[u][3]), b1 = cmul_si(csub(v[u][1], v[u][3]), sign);
+        v[u][0] = cadd(a0, b0);
+        v[u][2] = csub(a0, b0);
+        v[u][1] = cadd(a1, b1);
+        v[u][3] = csub(a1, b1);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < kMaxPer; ++u) {
+    int b = threadIdx.x + u * kBlock;
+    if (b < nbf) {
+      int g = b & (G - 1);
+      int j = b >> log2G;
+      int k = j & (Ns - 1);
+      int d = ((j >> log2Ns) << (log2Ns + log2R)) + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[((d + r * Ns) << log2G) + g] = v[u][r];
+    }
+  }
+  __syncthreads();
+}
+
+// G interleaved in-place transforms of 2^log2P points:
+//   out[k] = sum_j in[j] e^{sign 2 pi i j k / P}  (unnormalised)
+// Requires G * 2^log2P <= kLdsCap, G a power of two, blockDim.x == kBlock, and
+// a __syncthreads() between filling buf and the call.
+__device__ inline void lds_fft(float2* buf, int log2P, int G, int log2G, const Twiddles& tw, float sign) {
+  int log2Ns = 0;
+  if (log2P & 1) {
+    stockham_stage<2>(buf, log2P, 0, G, log2G, tw, sign);
+    log2Ns = 1;
+  }
+  for (; log2Ns < log2P; log2Ns += 2) stockham_stage<4>(buf, log2P, log2Ns, G, log2G, tw, sign);
+}
+
+}  // namespace fcwt_hip

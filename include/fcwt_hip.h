@@ -1,0 +1,149 @@
+/*
+ * fcwt_hip.h -- C ABI of the MI355X (gfx950) implementation of fCWT's
+ * FCWT::cwt hot path (Morlet wavelet, real float32 input).
+ *
+ * Plain C types only: pointers, sizes, status codes.  Every entry point
+ * returns FCWT_HIP_OK (0) or a negative status; fcwt_hip_last_error() then
+ * holds a message (thread-local).  Nothing throws across this boundary.
+ *
+ * Each declaration names the reference interface it replaces
+ * (fastlib/fCWT, src/fcwt/...).  A plan is not safe for concurrent use from
+ * several threads (the reference's FCWT object is not either: fcwt.cpp:450
+ * mutates this->size and the wavelet table during cwt).
+ */
+#ifndef FCWT_HIP_H
+#define FCWT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define FCWT_HIP_OK 0
+#define FCWT_HIP_EINVAL (-1)      /* invalid argument */
+#define FCWT_HIP_ENOMEM (-2)      /* host or device allocation failed */
+#define FCWT_HIP_EDEVICE (-3)     /* HIP runtime error, or no usable gfx950 device */
+#define FCWT_HIP_EUNSUPPORTED (-4) /* size outside what this build handles */
+#define FCWT_HIP_EIO (-5)          /* plan-cache file could not be read or written */
+
+/* Scale types; same values as enum SCALETYPE, src/fcwt/fcwt.h:65. */
+#define FCWT_HIP_LINSCALES 0
+#define FCWT_HIP_LOGSCALES 1
+#define FCWT_HIP_LINFREQS 2
+
+/* Plan flags. */
+/* Bins covered per scale follow the reference's scalar branch
+ * (src/fcwt/fcwt.cpp:276-280) instead of its AVX branch (fcwt.cpp:203-214,
+ * the branch every shipped build compiles). */
+#define FCWT_HIP_COVERAGE_SCALAR 0x1u
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* fcwt_hip_last_error(void);
+
+/* Library version string. */
+const char* fcwt_hip_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Host arithmetic of the reference, bit-exact (no device involved).       */
+/* ---------------------------------------------------------------------- */
+
+/* find2power, src/fcwt/fcwt.h:154-165: smallest m with 2^m >= n. */
+int fcwt_hip_find2power(int n);
+
+/* Scales::Scales(wavelet, st, fs, f0, f1, fn), src/fcwt/fcwt.cpp:111-190;
+ * writes fn float32 scales.  (The wavelet argument only feeds four_wavelen,
+ * which the scale formulas never use.) */
+int fcwt_hip_scales(int scaletype, int fs, float f0, float f1, int fn, float* scales);
+
+/* Scales::getFrequencies, src/fcwt/fcwt.cpp:133-137: freqs[i] = fs / scales[i]. */
+int fcwt_hip_frequencies(const float* scales, int fn, int fs, float* freqs);
+
+/* Morlet::generate(int size), src/fcwt/fcwt.cpp:51-67: frequency-domain
+ * mother table of `size` float32 values (host copy). */
+int fcwt_hip_morlet_generate(float fb, int size, float* mother);
+
+/* Morlet::getSupport, src/fcwt/fcwt.h:88: (int)(fb*scale*3.0f). */
+int fcwt_hip_morlet_support(float fb, float scale);
+
+/* Morlet::generate(real, imag, size, scale), src/fcwt/fcwt.cpp:68-84:
+ * writes the 2*support+1 time-domain taps (support = fcwt_hip_morlet_support)
+ * to real[] and imag[], normalised with `size` as the reference does. */
+int fcwt_hip_morlet_generate_time(float fb, float scale, int size, float* real, float* imag);
+
+/* Morlet::getWavelet(scale, pwav, pn), src/fcwt/fcwt.cpp:86-105: time-domain
+ * wavelet, pn interleaved complex64 values (first pn of the 2*support+1 taps). */
+int fcwt_hip_morlet_wavelet(float fb, float scale, float* wavelet, int pn);
+
+/* The spectrum each output row is the inverse DFT of, as planned (host only,
+ * no device): for scale i, bins [0, covered[i]) hold X[q]*mother[idx_i(q)]
+ * (fcwt.cpp:197-311) and bins up to band[i] may also hold the product an
+ * earlier scale left in the reference's reused O1 buffer (fcwt.cpp:449,
+ * 466-467, 514-518); segments[i] counts the distinct windows.  Products whose
+ * window is below 1e-12 of the window peak are left out of band[i].
+ * Any output pointer may be NULL. */
+int fcwt_hip_describe_rows(int n, const float* scales, int fn, float fb, int threads, unsigned flags,
+                           int* covered, int* band, int* segments);
+
+/* ---------------------------------------------------------------------- */
+/* Plans and execution on the GPU.                                         */
+/* ---------------------------------------------------------------------- */
+
+typedef struct fcwt_hip_plan_s* fcwt_hip_plan;
+
+/* Everything FCWT::cwt sets up per call before its scale loop
+ * (src/fcwt/fcwt.cpp:447-511: padded size N = 2^find2power(n), buffers,
+ * mother table wavelet->generate(N)), done once on the current HIP device:
+ * per-scale band metadata, mother table (float64 maths rounded to float32),
+ * twiddle tables, launch schedule.
+ *   n          signal length (>= 1)
+ *   scales     fn float32 scales (host pointer; copied)
+ *   fb         Morlet bandwidth (Morlet::Morlet(fb), fcwt.cpp:41-49)
+ *   threads    the FCWT constructor's thread count (fcwt.h:116); it only
+ *              selects how many bins per scale the reference covers
+ *              (fcwt.cpp:213-214 / 277-278), which the plan reproduces
+ *   normalize  use_normalization (fcwt.h:116): divide rows by N (fcwt.cpp:431-445)
+ *   flags      FCWT_HIP_* plan flags */
+int fcwt_hip_plan_create(fcwt_hip_plan* plan, int n, const float* scales, int fn, float fb, int threads,
+                         int normalize, unsigned flags);
+
+/* Frees the plan and its device memory. */
+int fcwt_hip_plan_destroy(fcwt_hip_plan plan);
+
+/* Plan facts: signal length, padded FFT size N, number of scales. */
+int fcwt_hip_plan_shape(fcwt_hip_plan plan, int* n, int* N, int* fn);
+
+/* Kernel timing for benchmarks and tuning (no reference counterpart; the
+ * reference times whole calls with std::chrono, benchmark.cpp:225-230).
+ * enable != 0 makes every later execute record HIP events around each kernel
+ * on its stream; the call also clears the totals. */
+int fcwt_hip_plan_profile(fcwt_hip_plan plan, int enable);
+
+/* Waits for the recorded events and reports, per kernel kind that ran
+ * ("forward_fft", "row_writer_direct", "row_spreader",
+ * "row_writer_two_pass"), the summed milliseconds and launch count since the
+ * last fcwt_hip_plan_profile.  Writes at most `max` entries; *count gets the
+ * number of kinds that ran. */
+int fcwt_hip_plan_profile_read(fcwt_hip_plan plan, int max, const char** names, double* ms, long long* launches,
+                               int* count);
+
+/* Bytes of device memory the plan holds (excluding caller buffers). */
+int fcwt_hip_plan_device_bytes(fcwt_hip_plan plan, size_t* bytes);
+
+/* FCWT::cwt(float* in, int n, complex<float>* out, Scales*),
+ * src/fcwt/fcwt.cpp:532-534 (core 447-529), host pointers: input holds n
+ * float32 samples, output receives fn*n interleaved complex64 values, row i
+ * (scale i) at output + 2*i*n.  Copies in, runs, copies out; returns when the
+ * output is complete. */
+int fcwt_hip_cwt(fcwt_hip_plan plan, const float* input, float* output);
+
+/* The same with device pointers, enqueued on `stream` (a hipStream_t; NULL =
+ * the default stream).  Returns once enqueued.  d_output holds fn*n complex64. */
+int fcwt_hip_cwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FCWT_HIP_H */

