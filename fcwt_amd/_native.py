@@ -43,6 +43,7 @@ SIGNATURES = {
     "fcwt_hip_plan_destroy": (ctypes.c_int, [_plan_t]),
     "fcwt_hip_plan_shape": (ctypes.c_int, [_plan_t, _c_int_p, _c_int_p, _c_int_p]),
     "fcwt_hip_plan_device_bytes": (ctypes.c_int, [_plan_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "fcwt_hip_plan_check": (ctypes.c_int, [_plan_t]),
     "fcwt_hip_plan_profile": (ctypes.c_int, [_plan_t, ctypes.c_int]),
     "fcwt_hip_plan_profile_read": (ctypes.c_int, [_plan_t, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                                   ctypes.POINTER(ctypes.c_double),

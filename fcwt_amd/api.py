@@ -164,6 +164,10 @@ class Plan:
         nat.check(nat.lib().fcwt_hip_plan_device_bytes(self._h, ctypes.byref(b)), "plan_device_bytes")
         return b.value
 
+    def check(self) -> None:
+        """Wait for the device; raise if an in-kernel synchronisation failed."""
+        nat.check(nat.lib().fcwt_hip_plan_check(self._h), "fcwt_hip_plan_check")
+
     def profile(self, enable: bool = True) -> None:
         """Start (and reset) or stop per-kernel event timing."""
         nat.check(nat.lib().fcwt_hip_plan_profile(self._h, int(bool(enable))), "plan_profile")

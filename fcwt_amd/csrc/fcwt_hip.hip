@@ -40,6 +40,7 @@ namespace fcwt_hip {
 constexpr int kMaxLog2B = 10;  // writer FFT size cap: G * B <= kLdsCap with G >= 16
 constexpr double kStaleFloor = 1e-12;
 constexpr size_t kScratchBytes = size_t(256) << 20;
+constexpr int kKinds = 5;  // kernel kinds timed by fcwt_hip_plan_profile
 
 struct SegD {
   int lo, hi;
@@ -75,6 +76,30 @@ struct KArgs {
   float norm;
   Twiddles tw;
 };
+
+}  // namespace fcwt_hip
+
+#include "rows_direct.h"
+#include "rows_team.h"
+
+namespace fcwt_hip {
+
+// phases per work-group of k_rows_direct<log2M> (before clamping to L)
+static int direct_phases(int log2M) {
+  switch (log2M) {
+    case 0: return DirectCfg<0>::G;
+    case 1: return DirectCfg<1>::G;
+    case 2: return DirectCfg<2>::G;
+    case 3: return DirectCfg<3>::G;
+    case 4: return DirectCfg<4>::G;
+    case 5: return DirectCfg<5>::G;
+    case 6: return DirectCfg<6>::G;
+    case 7: return DirectCfg<7>::G;
+    case 8: return DirectCfg<8>::G;
+    case 9: return DirectCfg<9>::G;
+    default: return DirectCfg<10>::G;
+  }
+}
 
 // Z[q] = X[q] * mother[(int)fminf(N-1, step*q)] for the segment holding q
 // (fcwt.cpp:224-248), 0 outside every segment.
@@ -307,7 +332,22 @@ struct fcwt_hip_plan_s {
   std::vector<RowD> rows;
   std::vector<SegD> segs;
   std::vector<Item> items;  // direct writer items, then per batch: spreader items, writer items
+  // XCD-team path for wide bands (rows_team.h)
+  bool team_capable = false;
+  int team_size = 0, teams_per_xcd = 0, nteams = 0, team_log2F = 0;
+  std::vector<int> team_rows, team_count;
+  int* d_team_rows = nullptr;
+  int* d_team_count = nullptr;
+  float2* d_team_scratch = nullptr;
+  unsigned* d_team_sync = nullptr;
+  unsigned* d_error = nullptr;
+  size_t team_sync_words = 0;
   size_t direct_items = 0;
+  struct Group {
+    int log2M;
+    size_t begin, count;
+  };
+  std::vector<Group> direct_groups;
   struct Batch {
     size_t s_begin, s_count, w_begin, w_count;
   };
@@ -338,12 +378,12 @@ struct fcwt_hip_plan_s {
   bool profiling = false;
   std::vector<Timed> pending;
   std::vector<hipEvent_t> event_pool;
-  double prof_ms[4] = {0, 0, 0, 0};
-  long long prof_launches[4] = {0, 0, 0, 0};
+  double prof_ms[kKinds] = {};
+  long long prof_launches[kKinds] = {};
 };
 
 // kernel kinds reported by fcwt_hip_plan_profile_read
-static const char* const kKindNames[4] = {"forward_fft", "row_writer_direct", "row_spreader", "row_writer_two_pass"};
+static const char* const kKindNames[kKinds] = {"forward_fft", "row_writer_direct", "row_spreader", "row_writer_two_pass", "row_team"};
 
 static hipEvent_t take_event(fcwt_hip_plan p) {
   if (!p->event_pool.empty()) {
@@ -396,7 +436,8 @@ static void free_plan(fcwt_hip_plan p) {
   drain_profile(p);
   for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
-                  p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out};
+                  p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
+                  p->d_team_rows, p->d_team_count, p->d_team_scratch, p->d_team_sync, p->d_error};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -431,7 +472,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   std::vector<RowBand> bands =
       build_bands(p->N, p->scales.data(), p->fn, p->threads, avx, mother_host.data(), kStaleFloor);
   p->rows.resize(p->fn);
-  std::vector<int> wide;
+  std::vector<int> wide, team;
+  const int team_log2F = p->log2N - kTeamLog2B;
+  const bool team_ok = p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F;
   for (int i = 0; i < p->fn; ++i) {
     RowD& r = p->rows[i];
     r.seg_begin = (int)p->segs.size();
@@ -439,20 +482,46 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     for (const Segment& s : bands[i].segs) p->segs.push_back({s.lo, s.hi, s.step, 0});
     int log2M = find2power(std::max(bands[i].band, 1));
     log2M = std::min(log2M, p->log2N);
-    r.log2B = std::min(log2M, kMaxLog2B);
-    r.log2A = log2M - r.log2B;
+    if (log2M > kMaxLog2B && team_ok && r.seg_count <= kTeamSegLds) {
+      r.log2B = kTeamLog2B;
+      r.log2A = log2M - kTeamLog2B;
+      team.push_back(i);
+    } else {
+      r.log2B = std::min(log2M, kMaxLog2B);
+      r.log2A = log2M - r.log2B;
+      if (r.log2A > 0) wide.push_back(i);
+    }
     int log2F = p->log2N - r.log2B;
     r.log2G = std::min(log2F, 14 - r.log2B);
     r.slot = -1;
     r.out_off = (long long)i * p->n;
-    if (r.log2A > 0) wide.push_back(i);
   }
-  // direct writer items
-  for (int i = 0; i < p->fn; ++i) {
-    const RowD& r = p->rows[i];
-    if (r.log2A != 0) continue;
-    int nblk = 1 << (p->log2N - r.log2B - r.log2G);
-    for (int b = 0; b < nblk; ++b) p->items.push_back({i, b});
+  if (!team.empty()) {
+    int items = 1 << (team_log2F - 4);
+    p->team_size = std::min(32, items);
+    p->teams_per_xcd = 32 / p->team_size;
+    p->nteams = 8 * p->teams_per_xcd;
+    p->team_log2F = team_log2F;
+    int per = ((int)team.size() + p->nteams - 1) / p->nteams;
+    p->team_rows.assign((size_t)per * p->nteams, 0);
+    p->team_count.assign(p->nteams, 0);
+    for (size_t k = 0; k < team.size(); ++k) {
+      int t = (int)(k % p->nteams), i = (int)(k / p->nteams);
+      p->team_rows[t + (size_t)p->nteams * i] = team[k];
+      p->team_count[t] = i + 1;
+    }
+  }
+  // direct items, grouped by band size (one kernel instance per size)
+  for (int lm = 0; lm <= kMaxLog2B; ++lm) {
+    size_t begin = p->items.size();
+    int L = 1 << (p->log2N - lm);
+    int G = std::min(direct_phases(lm), L);
+    for (int i = 0; i < p->fn; ++i) {
+      const RowD& r = p->rows[i];
+      if (r.log2A != 0 || r.log2B != lm) continue;
+      for (int b = 0; b < L / G; ++b) p->items.push_back({i, b});
+    }
+    if (p->items.size() > begin) p->direct_groups.push_back({lm, begin, p->items.size() - begin});
   }
   p->direct_items = p->items.size();
   if (!wide.empty()) {
@@ -496,6 +565,25 @@ static int check_device(int* dev) {
   return FCWT_HIP_OK;
 }
 
+// The team path launches one work-group per CU cooperatively and forms teams
+// of the 32 work-groups each XCD receives: it needs the full 256-CU device in
+// one partition.  FCWT_HIP_NO_TEAMS=1 in the environment turns it off.
+static bool team_capable(int dev) {
+  const char* env = std::getenv("FCWT_HIP_NO_TEAMS");
+  if (env && env[0] == '1') return false;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
+  if (prop.multiProcessorCount != 256 || !prop.cooperativeLaunch) return false;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rows_team, kTeamThreads, 0) != hipSuccess) return false;
+  return per_cu >= 1;
+}
+
+__global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 #define TRY(expr)                 \
   do {                            \
     int rc_ = (expr);             \
@@ -504,6 +592,27 @@ static int check_device(int* dev) {
       return rc_;                 \
     }                             \
   } while (0)
+
+template <int LOG2M>
+static void launch_direct_t(unsigned blocks, const KArgs& a, hipStream_t st) {
+  k_rows_direct<LOG2M><<<blocks, DirectCfg<LOG2M>::NT, 0, st>>>(a);
+}
+
+static void launch_direct(int log2M, unsigned blocks, const KArgs& a, hipStream_t st) {
+  switch (log2M) {
+    case 0: launch_direct_t<0>(blocks, a, st); break;
+    case 1: launch_direct_t<1>(blocks, a, st); break;
+    case 2: launch_direct_t<2>(blocks, a, st); break;
+    case 3: launch_direct_t<3>(blocks, a, st); break;
+    case 4: launch_direct_t<4>(blocks, a, st); break;
+    case 5: launch_direct_t<5>(blocks, a, st); break;
+    case 6: launch_direct_t<6>(blocks, a, st); break;
+    case 7: launch_direct_t<7>(blocks, a, st); break;
+    case 8: launch_direct_t<8>(blocks, a, st); break;
+    case 9: launch_direct_t<9>(blocks, a, st); break;
+    default: launch_direct_t<10>(blocks, a, st); break;
+  }
+}
 
 static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStream_t st) {
   // forward transform
@@ -544,8 +653,28 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStrea
   a.tw = twiddles_of(p);
   if (p->direct_items) {
     ProfScope ps(p, 1, st);
-    a.items = p->d_items;
-    k_writer<<<(unsigned)p->direct_items, kBlock, 0, st>>>(a);
+    for (const auto& gr : p->direct_groups) {
+      a.items = p->d_items + gr.begin;
+      launch_direct(gr.log2M, (unsigned)gr.count, a, st);
+    }
+  }
+  if (p->nteams) {
+    ProfScope ps(p, 4, st);
+    size_t words = p->team_sync_words;
+    k_fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(p->d_team_sync, words, 0u);
+    TeamArgs ta;
+    ta.k = a;
+    ta.team_rows = p->d_team_rows;
+    ta.team_count = p->d_team_count;
+    ta.nteams = p->nteams;
+    ta.team_size = p->team_size;
+    ta.teams_per_xcd = p->teams_per_xcd;
+    ta.log2F = p->team_log2F;
+    ta.scratch = p->d_team_scratch;
+    ta.sync = p->d_team_sync;
+    ta.error = p->d_error;
+    void* args[] = {&ta};
+    HIP_TRY(hipLaunchCooperativeKernel((const void*)k_rows_team, dim3(256), dim3(kTeamThreads), args, 0, st));
   }
   for (const auto& bt : p->batches) {
     {
@@ -649,6 +778,7 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   p->threads = threads;
   p->normalize = normalize != 0;
   p->flags = flags;
+  p->team_capable = team_capable(dev);
   p->scales.assign(sc, sc + fn);
 
   // host copy of the mother table, used only to judge which stale bins matter
@@ -678,6 +808,14 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(dalloc(p, &p->d_rows, p->rows.size()));
   TRY(dalloc(p, &p->d_items, p->items.size()));
   if (p->scratch_slots) TRY(dalloc(p, &p->d_scratch, (size_t)p->scratch_slots * N));
+  TRY(dalloc(p, &p->d_error, 1));
+  if (p->nteams) {
+    TRY(dalloc(p, &p->d_team_rows, p->team_rows.size()));
+    TRY(dalloc(p, &p->d_team_count, p->team_count.size()));
+    TRY(dalloc(p, &p->d_team_scratch, (size_t)p->nteams * kTeamBuffers * N));
+    p->team_sync_words = (size_t)64 * (1 + p->nteams * kTeamBuffers);
+    TRY(dalloc(p, &p->d_team_sync, p->team_sync_words));
+  }
 
   auto up = [&](void* dst, const void* src, size_t bytes) -> int {
     if (bytes == 0) return FCWT_HIP_OK;
@@ -687,6 +825,11 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(up(p->d_segs, p->segs.data(), p->segs.size() * sizeof(SegD)));
   TRY(up(p->d_rows, p->rows.data(), p->rows.size() * sizeof(RowD)));
   TRY(up(p->d_items, p->items.data(), p->items.size() * sizeof(Item)));
+  if (p->nteams) {
+    TRY(up(p->d_team_rows, p->team_rows.data(), p->team_rows.size() * sizeof(int)));
+    TRY(up(p->d_team_count, p->team_count.data(), p->team_count.size() * sizeof(int)));
+  }
+  k_fill_u32<<<1, 64, 0, p->stream>>>(p->d_error, 1, 0u);
 
   // mother-table scalars in float32 exactly as fcwt.cpp:56-57 forms them
   const float kPi = 3.14159265358979323846264338327950288f;
@@ -716,11 +859,35 @@ int fcwt_hip_plan_shape(fcwt_hip_plan p, int* n, int* N, int* fn) {
   return FCWT_HIP_OK;
 }
 
+static int read_error_word(fcwt_hip_plan p) {
+  unsigned h = 0;
+  HIP_TRY(hipMemcpy(&h, p->d_error, sizeof(unsigned), hipMemcpyDeviceToHost));
+  if (h != 0) {
+    unsigned zero = 0;
+    HIP_TRY(hipMemcpy(p->d_error, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
+    return fail(FCWT_HIP_EDEVICE, h & 2u ? "wide-band kernel: work-groups were not spread 32 per XCD"
+                                         : "wide-band kernel: a team wait timed out");
+  }
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_check(fcwt_hip_plan p) {
+  if (!p) return fail(FCWT_HIP_EINVAL, "null plan");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  hipError_t e = hipDeviceSynchronize();
+  int rc = e == hipSuccess ? read_error_word(p)
+                           : fail(FCWT_HIP_EDEVICE, std::string("device error: ") + hipGetErrorString(e));
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
+}
+
 int fcwt_hip_plan_profile(fcwt_hip_plan p, int enable) {
   if (!p) return fail(FCWT_HIP_EINVAL, "null plan");
   drain_profile(p);
   p->profiling = enable != 0;
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kKinds; ++k) {
     p->prof_ms[k] = 0;
     p->prof_launches[k] = 0;
   }
@@ -732,7 +899,7 @@ int fcwt_hip_plan_profile_read(fcwt_hip_plan p, int max, const char** names, dou
   if (!p || max < 0) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_profile_read: bad arguments");
   drain_profile(p);
   int k = 0;
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < kKinds; ++i) {
     if (p->prof_launches[i] == 0) continue;
     if (k < max) {
       if (names) names[k] = kKindNames[i];
@@ -780,6 +947,7 @@ int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("cwt failed: ") + hipGetErrorString(e));
   }
+  if (rc == FCWT_HIP_OK && p->nteams) rc = read_error_word(p);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
 }

@@ -12,6 +12,21 @@
 
 namespace fcwt_hip {
 
+// float2 in the global address space: pointer parameters of non-inlined
+// device functions keep global_* instructions instead of flat_* ones.
+typedef __attribute__((address_space(1))) float2 gfloat2;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+
+// *p = v through a global-address-space pointer (one 8-byte store)
+__device__ __forceinline__ void gstore(gfloat2* p, float2 v) {
+  *reinterpret_cast<gu64*>(p) = ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
+}
+// the same with the non-temporal hint, for output that is written once and not re-read
+__device__ __forceinline__ void gstore_nt(gfloat2* p, float2 v) {
+  __builtin_nontemporal_store(((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x),
+                              reinterpret_cast<gu64*>(p));
+}
+
 constexpr int kLdsCap = 16384;  // complex64 elements per work-group (128 KiB)
 constexpr int kBlock = 1024;    // threads per LDS-FFT work-group
 

@@ -1,0 +1,154 @@
+// rows_direct.h -- the fused window-multiply + band-limited inverse FFT for
+// scales whose band fits one work-group (M = 2^LOG2M <= 1024 bins).
+// Device code, included by fcwt_hip.hip.
+//
+// Row y[t] = c * sum_{q<M} Z[q] e^{2 pi i q t / N} is split into L = N/M
+// phases l:  y[l + L*m] = c * IDFT_M(Z[q] * W_N^{q*l})[m].  A work-group takes
+// G consecutive phases, so for every m it owns G consecutive outputs and the
+// row is written in runs of G complex64 (G >= 16: 128-byte runs).
+//
+// Each phase's M-point transform is M = R1 * P2 (R1 <= 32): P2 threads each
+// hold R1 points q = j + P2*k in registers.  With W_N^{q l} =
+// W_N^{j l} * W_{R1 L}^{k l}, the second factor is applied before the R1-point
+// DFT (a per-block table of R1 x G values) and the first, being constant in k,
+// after it, folded into the inter-pass twiddle: W_N^{j l} W_M^{j k1} =
+// W_N^{j (l + L k1)}.  One LDS exchange, the P2-point DFTs, then an LDS
+// transpose into [m][g] order for the stores.
+//
+// LDS holds, besides the exchange/transpose buffer: the row's windowed band
+// Z[0, M) (shared by all G phases), and the tables W_M^r (r < M) and
+// W_{R1 L}^{k l}.
+#pragma once
+#include "fft_lds.h"
+#include "fft_reg.h"
+
+namespace fcwt_hip {
+
+template <int LOG2M>
+struct DirectCfg {
+  static constexpr int M = 1 << LOG2M;
+  static constexpr int R1 = M < 32 ? M : 32;
+  static constexpr int P2 = M / R1;
+  // phases per work-group (upper bound; fewer when the row has fewer phases)
+  static constexpr int G = P2 >= 16 ? 16 : 256 / P2;
+  static constexpr int NT = G * P2;                // threads
+  static constexpr int XS = R1 * (P2 + 1);         // LDS stride between phases in the exchange
+  static constexpr int OS = G + 1;                 // LDS stride between m rows in the transpose
+  static constexpr int BUF = (P2 > 1 && G * XS > M * OS) ? G * XS : M * OS;
+  static constexpr int LDS = BUF + M /*Z*/ + M /*W_M*/ + R1 * G /*pre-twiddles*/;
+};
+
+// Z[q] for one row: X[q] * mother[(int)fminf(N-1, step*q)] in the segment
+// holding q (fcwt.cpp:224-248), 0 outside every segment.
+struct RowSpectrum {
+  const float2* spec;
+  const float* mother;
+  const SegD* segs;
+  int seg_begin, seg_count;
+  float maxidx;
+  __device__ __forceinline__ float2 operator()(int q) const {
+#pragma clang fp contract(off)
+    for (int s = 0; s < seg_count; ++s) {
+      SegD sg = segs[seg_begin + s];
+      if (q >= sg.lo && q < sg.hi) {
+        float v = sg.step * (float)q;
+        int idx = (int)(v < maxidx ? v : maxidx);
+        float w = mother[idx];
+        float2 x = spec[q];
+        return make_float2(x.x * w, x.y * w);
+      }
+    }
+    return make_float2(0.f, 0.f);
+  }
+};
+
+template <int LOG2M>
+__global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
+  using C = DirectCfg<LOG2M>;
+  constexpr int M = C::M, R1 = C::R1, P2 = C::P2, NT = C::NT;
+  __shared__ float2 lds[C::LDS];
+  float2* const buf = lds;
+  float2* const zs = lds + C::BUF;     // Z[0, M)
+  float2* const wm = zs + M;           // W_M^r
+  float2* const pre = wm + M;          // [g][k] = W_{R1 L}^{k l}
+  const Item it = a.items[blockIdx.x];
+  const RowD r = a.rows[it.row];
+  const int log2L = a.log2N - LOG2M;
+  const int L = 1 << log2L;
+  const int G = min(C::G, L);  // phases in this block
+  const int l0 = it.block * G;
+  const int tid = threadIdx.x;
+  {
+    RowSpectrum Z{a.spec, a.mother, a.segs, r.seg_begin, r.seg_count, (float)a.N - 1.0f};
+    for (int q = tid; q < M; q += NT) {
+      zs[q] = Z(q);
+      wm[q] = a.tw((unsigned)q << log2L);
+    }
+    // W_{R1 L}^{k l} = W_N^{k l P2}
+    for (int e = tid; e < R1 * G; e += NT) {
+      int g = e / R1, k = e - g * R1;
+      pre[e] = a.tw(((unsigned)(k * (l0 + g)) * P2) & (unsigned)(a.N - 1));
+    }
+  }
+  __syncthreads();
+  const int g = tid / P2, j = tid % P2;
+  const bool active = g < G;
+  float2 v[R1];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < R1; ++k) v[k] = cmul(zs[j + P2 * k], pre[g * R1 + k]);
+    dft_reg<R1, 1>(v);
+  }
+  if constexpr (P2 > 1) {
+    if (active) {
+      // W_N^{j l} W_M^{j k1}; W_N^{j l} = W_M^{j l'} * W_N^{j (l mod L')}... computed once
+      const unsigned l = (unsigned)(l0 + g);
+      const float2 base = a.tw((unsigned)(j * l) & (unsigned)(a.N - 1));
+      v[0] = cmul(v[0], base);
+#pragma unroll
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[(j * k1) & (M - 1)]));
+#pragma unroll
+      for (int k1 = 0; k1 < R1; ++k1) buf[g * C::XS + k1 * (P2 + 1) + j] = v[k1];
+    }
+    __syncthreads();
+    if (active) {
+      constexpr int U = R1 / P2;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k1 = j * U + u;
+        float2 w[P2];
+#pragma unroll
+        for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * C::XS + k1 * (P2 + 1) + jj];
+        dft_reg<P2, 1>(w);
+#pragma unroll
+        for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
+      }
+    }
+    __syncthreads();
+  } else {
+    // single thread per phase: only the phase factor W_N^{q l} = W_{R1 L}^{k l} applies (j = 0)
+  }
+  // transpose through LDS to [m][g]; m = k1 + R1*k2
+  if (active) {
+    if constexpr (P2 > 1) {
+      constexpr int U = R1 / P2;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k2 = 0; k2 < P2; ++k2) buf[((j * U + u) + R1 * k2) * C::OS + g] = cscale(v[u * P2 + k2], a.norm);
+    } else {
+#pragma unroll
+      for (int m = 0; m < R1; ++m) buf[m * C::OS + g] = cscale(v[m], a.norm);
+    }
+  }
+  __syncthreads();
+  float2* out = a.out + r.out_off;
+  const int tot = M * G;
+  for (int e = tid; e < tot; e += NT) {
+    int m = e / G, gg = e - m * G;
+    long long t = (long long)(l0 + gg) + ((long long)m << log2L);
+    if (t < a.n) out[t] = buf[m * C::OS + gg];
+  }
+}
+
+}  // namespace fcwt_hip

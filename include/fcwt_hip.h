@@ -114,6 +114,14 @@ int fcwt_hip_plan_destroy(fcwt_hip_plan plan);
 /* Plan facts: signal length, padded FFT size N, number of scales. */
 int fcwt_hip_plan_shape(fcwt_hip_plan plan, int* n, int* N, int* fn);
 
+/* Waits for the device and reports whether the last executes of this plan
+ * completed their in-kernel synchronisation (FCWT_HIP_EDEVICE if a
+ * wide-band team wait failed; the output is then invalid).  fcwt_hip_cwt
+ * performs this check itself; callers of fcwt_hip_cwt_device call this after
+ * their stream work completes.  No reference counterpart (the reference has
+ * no asynchronous entry point). */
+int fcwt_hip_plan_check(fcwt_hip_plan plan);
+
 /* Kernel timing for benchmarks and tuning (no reference counterpart; the
  * reference times whole calls with std::chrono, benchmark.cpp:225-230).
  * enable != 0 makes every later execute record HIP events around each kernel

@@ -36,4 +36,5 @@ def run_device(x, scales, fb, threads=1, normalize=True, flags=0, plan=None):
     out = torch.empty((sc.size, x.size), dtype=torch.complex64, device="cuda:0")
     p.execute_device(dx.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
+    p.check()
     return out.cpu().numpy()
