@@ -334,7 +334,7 @@ struct fcwt_hip_plan_s {
   std::vector<Item> items;  // direct writer items, then per batch: spreader items, writer items
   // XCD-team path for wide bands (rows_team.h)
   bool team_capable = false;
-  int team_size = 0, teams_per_xcd = 0, nteams = 0, team_log2F = 0;
+  int team_size = 0, teams_per_xcd = 0, nteams = 0, team_log2F = 0, team_nbuf = kTeamBuffers;
   std::vector<int> team_rows, team_count;
   int* d_team_rows = nullptr;
   int* d_team_count = nullptr;
@@ -499,7 +499,12 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   if (!team.empty()) {
     int items = 1 << (team_log2F - 4);
     p->team_size = std::min(32, items);
-    p->teams_per_xcd = 32 / p->team_size;
+    p->teams_per_xcd = kTeamWgPerXcd / p->team_size;
+    // scratch buffers per team (pipeline depth); FCWT_HIP_TEAM_BUFFERS overrides for tuning
+    if (const char* e = std::getenv("FCWT_HIP_TEAM_BUFFERS")) {
+      int v = std::atoi(e);
+      if (v >= 1 && v <= kTeamBuffers) p->team_nbuf = v;
+    }
     p->nteams = 8 * p->teams_per_xcd;
     p->team_log2F = team_log2F;
     int per = ((int)team.size() + p->nteams - 1) / p->nteams;
@@ -576,7 +581,7 @@ static bool team_capable(int dev) {
   if (prop.multiProcessorCount != 256 || !prop.cooperativeLaunch) return false;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rows_team, kTeamThreads, 0) != hipSuccess) return false;
-  return per_cu >= 1;
+  return per_cu >= kTeamWgPerCu;
 }
 
 __global__ void k_fill_u32(unsigned* p, size_t n, unsigned v) {
@@ -670,11 +675,13 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStrea
     ta.team_size = p->team_size;
     ta.teams_per_xcd = p->teams_per_xcd;
     ta.log2F = p->team_log2F;
+    ta.nbuf = p->team_nbuf;
     ta.scratch = p->d_team_scratch;
     ta.sync = p->d_team_sync;
     ta.error = p->d_error;
     void* args[] = {&ta};
-    HIP_TRY(hipLaunchCooperativeKernel((const void*)k_rows_team, dim3(256), dim3(kTeamThreads), args, 0, st));
+    HIP_TRY(hipLaunchCooperativeKernel((const void*)k_rows_team, dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args,
+                                       0, st));
   }
   for (const auto& bt : p->batches) {
     {

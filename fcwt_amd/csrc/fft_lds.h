@@ -21,6 +21,11 @@ typedef __attribute__((address_space(1))) unsigned long long gu64;
 __device__ __forceinline__ void gstore(gfloat2* p, float2 v) {
   *reinterpret_cast<gu64*>(p) = ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x);
 }
+// *p through a global-address-space pointer (one 8-byte load)
+__device__ __forceinline__ float2 gload(const gfloat2* p) {
+  unsigned long long u = *reinterpret_cast<const gu64*>(p);
+  return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+}
 // the same with the non-temporal hint, for output that is written once and not re-read
 __device__ __forceinline__ void gstore_nt(gfloat2* p, float2 v) {
   __builtin_nontemporal_store(((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x),

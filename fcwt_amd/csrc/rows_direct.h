@@ -11,13 +11,15 @@
 // hold R1 points q = j + P2*k in registers.  With W_N^{q l} =
 // W_N^{j l} * W_{R1 L}^{k l}, the second factor is applied before the R1-point
 // DFT (a per-block table of R1 x G values) and the first, being constant in k,
-// after it, folded into the inter-pass twiddle: W_N^{j l} W_M^{j k1} =
-// W_N^{j (l + L k1)}.  One LDS exchange, the P2-point DFTs, then an LDS
-// transpose into [m][g] order for the stores.
+// after it, together with the inter-pass twiddle W_M^{j k1}.  One LDS
+// exchange, the P2-point DFTs, then an LDS transpose into [m][g] order for the
+// stores.
 //
 // LDS holds, besides the exchange/transpose buffer: the row's windowed band
-// Z[0, M) (shared by all G phases), and the tables W_M^r (r < M) and
-// W_{R1 L}^{k l}.
+// Z[0, M) (shared by all G phases), and the tables W_M^{j k1} as [k1][j] and
+// W_{R1 L}^{k l} as [k][g], so that the lanes of one instruction read
+// consecutive or identical entries.  The exchange is [g][j][k1] with stride
+// R1 + 1 per j, which keeps both its writes and reads on distinct banks.
 #pragma once
 #include "fft_lds.h"
 #include "fft_reg.h"
@@ -32,10 +34,10 @@ struct DirectCfg {
   // phases per work-group (upper bound; fewer when the row has fewer phases)
   static constexpr int G = P2 >= 16 ? 16 : 256 / P2;
   static constexpr int NT = G * P2;                // threads
-  static constexpr int XS = R1 * (P2 + 1);         // LDS stride between phases in the exchange
+  static constexpr int XS = M + P2;                // LDS stride between phases in the exchange
   static constexpr int OS = G + 1;                 // LDS stride between m rows in the transpose
   static constexpr int BUF = (P2 > 1 && G * XS > M * OS) ? G * XS : M * OS;
-  static constexpr int LDS = BUF + M /*Z*/ + M /*W_M*/ + R1 * G /*pre-twiddles*/;
+  static constexpr int LDS = BUF + M /*Z*/ + M /*W_M^{j k1}*/ + R1 * G /*pre-twiddles*/;
 };
 
 // Z[q] for one row: X[q] * mother[(int)fminf(N-1, step*q)] in the segment
@@ -69,8 +71,8 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   __shared__ float2 lds[C::LDS];
   float2* const buf = lds;
   float2* const zs = lds + C::BUF;     // Z[0, M)
-  float2* const wm = zs + M;           // W_M^r
-  float2* const pre = wm + M;          // [g][k] = W_{R1 L}^{k l}
+  float2* const wm = zs + M;           // [k1][j] = W_M^{j k1}
+  float2* const pre = wm + M;          // [k][g] = W_{R1 L}^{k l}
   const Item it = a.items[blockIdx.x];
   const RowD r = a.rows[it.row];
   const int log2L = a.log2N - LOG2M;
@@ -82,12 +84,15 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
     RowSpectrum Z{a.spec, a.mother, a.segs, r.seg_begin, r.seg_count, (float)a.N - 1.0f};
     for (int q = tid; q < M; q += NT) {
       zs[q] = Z(q);
-      wm[q] = a.tw((unsigned)q << log2L);
+      if (P2 > 1) {
+        const int j = q % P2, k1 = q / P2;  // j k1 < M
+        wm[q] = a.tw((unsigned)(j * k1) << log2L);
+      }
     }
     // W_{R1 L}^{k l} = W_N^{k l P2}
-    for (int e = tid; e < R1 * G; e += NT) {
-      int g = e / R1, k = e - g * R1;
-      pre[e] = a.tw(((unsigned)(k * (l0 + g)) * P2) & (unsigned)(a.N - 1));
+    for (int e = tid; e < R1 * C::G; e += NT) {
+      int k = e / C::G, g = e - k * C::G;
+      if (g < G) pre[e] = a.tw(((unsigned)(k * (l0 + g)) * P2) & (unsigned)(a.N - 1));
     }
   }
   __syncthreads();
@@ -96,29 +101,29 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   float2 v[R1];
   if (active) {
 #pragma unroll
-    for (int k = 0; k < R1; ++k) v[k] = cmul(zs[j + P2 * k], pre[g * R1 + k]);
+    for (int k = 0; k < R1; ++k) v[k] = cmul(zs[j + P2 * k], pre[k * C::G + g]);
     dft_reg<R1, 1>(v);
   }
   if constexpr (P2 > 1) {
     if (active) {
-      // W_N^{j l} W_M^{j k1}; W_N^{j l} = W_M^{j l'} * W_N^{j (l mod L')}... computed once
+      // W_N^{j l} (constant in k1) times W_M^{j k1}
       const unsigned l = (unsigned)(l0 + g);
       const float2 base = a.tw((unsigned)(j * l) & (unsigned)(a.N - 1));
       v[0] = cmul(v[0], base);
 #pragma unroll
-      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[(j * k1) & (M - 1)]));
+      for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[k1 * P2 + j]));
 #pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) buf[g * C::XS + k1 * (P2 + 1) + j] = v[k1];
+      for (int k1 = 0; k1 < R1; ++k1) buf[g * C::XS + j * (R1 + 1) + k1] = v[k1];
     }
     __syncthreads();
     if (active) {
       constexpr int U = R1 / P2;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int k1 = j * U + u;
+        const int k1 = u * P2 + j;
         float2 w[P2];
 #pragma unroll
-        for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * C::XS + k1 * (P2 + 1) + jj];
+        for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * C::XS + jj * (R1 + 1) + k1];
         dft_reg<P2, 1>(w);
 #pragma unroll
         for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int k2 = 0; k2 < P2; ++k2) buf[((j * U + u) + R1 * k2) * C::OS + g] = cscale(v[u * P2 + k2], a.norm);
+        for (int k2 = 0; k2 < P2; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = cscale(v[u * P2 + k2], a.norm);
     } else {
 #pragma unroll
       for (int m = 0; m < R1; ++m) buf[m * C::OS + g] = cscale(v[m], a.norm);

@@ -4,20 +4,27 @@
 // Device code, included by fcwt_hip.hip.
 //
 // With N = B * F (B = 256) and the band M = A * B:
-//   y[f + F*m] = c * sum_{qb<B} W_B^{qb m} U[qb][f]
-//   U[qb][f]   = W_N^{qb f} * sum_{qf<A} Z[qb + B qf] W_F^{qf f}
-// Pass 1 (per team member: Qb consecutive qb): the F-point sums, done as F/A
-// phases l' of A-point inverse FFTs (f = l' + (F/A) m'), into a scratch
-// buffer U[f][qb] in global memory.  Pass 2 (per member: 16 consecutive f):
-// B-point inverse FFTs over qb, written as 16-element (128-byte) runs.
+//   y[f + F*m] = c * sum_{qb<B} W_B^{qb m} W_N^{qb f} U[f][qb]
+//   U[f][qb]   = sum_{qf<A} Z[qb + B qf] W_F^{qf f}
+// Pass 1 (per team member: Qb = 4096/F consecutive qb): the F-point sums, done
+// as F/A phases l' of A-point inverse FFTs (f = l' + (F/A) m'), into a
+// scratch buffer U[f][qb] in global memory.  Pass 2 (per member: 16
+// consecutive f): the factor W_N^{qb f} and the B-point inverse FFTs over qb,
+// written as 16-element (128-byte) runs of the output row.
 //
 // All members of a team run on one XCD (chosen by the hardware XCC id), so
 // the scratch never leaves that XCD's L2: pass 1 stores normally, waits for
 // its stores (s_waitcnt), then bumps a per-buffer counter; pass 2 waits for
 // the counter and reads with device-scope loads, which bypass the CU's L1 and
-// are served by the shared L2.  Two scratch buffers per team let pass 1 of
-// one row overlap pass 2 of the previous row.  Every wait is bounded: on
-// timeout the kernel records an error and every work-group exits.
+// are served by the shared L2.  With two scratch buffers per team, pass 1 of
+// one row overlaps pass 2 of the previous one; two work-groups per CU (two
+// teams per XCD) interleave.  Every wait is bounded: on timeout the kernel
+// records an error and every work-group exits.
+//
+// Twiddle tables in LDS are laid out so that the lanes of one instruction
+// read consecutive or identical entries (no bank conflicts): pass-2 factors
+// W_B^{j k1} as [k1][j]; pass-1 factors as [k][l'] and [k1][l'][j]; padded
+// strides on the staged band, the exchanges and the tile.
 #pragma once
 #include "fft_lds.h"
 #include "fft_reg.h"
@@ -27,6 +34,8 @@ namespace fcwt_hip {
 constexpr int kTeamLog2B = 8;                  // B = 256
 constexpr int kTeamB = 1 << kTeamLog2B;
 constexpr int kTeamThreads = 256;
+constexpr int kTeamWgPerCu = 2;                // two teams interleave on every CU
+constexpr int kTeamWgPerXcd = 32 * kTeamWgPerCu;
 constexpr int kTeamPoints = 4096;              // points per member per pass
 constexpr int kTeamBuffers = 2;
 constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
@@ -42,7 +51,8 @@ struct TeamArgs {
   int team_size;           // work-groups per team
   int teams_per_xcd;
   int log2F;
-  float2* scratch;         // [team][buffer][N]
+  int nbuf;                // scratch buffers in use per team (1..kTeamBuffers)
+  float2* scratch;         // [team][kTeamBuffers][N]
   unsigned* sync;          // zeroed before launch; see below
   unsigned* error;         // set to a nonzero code on a failed wait or team formation
 };
@@ -59,8 +69,7 @@ __device__ __forceinline__ void add_agent(unsigned* p, unsigned v) {
 // Device-scope load of one complex64 (global_load_dwordx2 ... sc1): misses the
 // CU's L1, served by the XCD's L2.
 __device__ __forceinline__ float2 ld_agent2(const gfloat2* p) {
-  typedef const __attribute__((address_space(1))) unsigned long long gu64;
-  unsigned long long u = __hip_atomic_load(reinterpret_cast<gu64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned long long u = __hip_atomic_load(reinterpret_cast<const gu64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
 }
 
@@ -94,100 +103,107 @@ __device__ __forceinline__ float window_at(int q, int nseg, const SegD* sseg, co
   return mother[idx];
 }
 
+// Pass-1 register split of the A-point transform: R1 = min(A, 16) points per
+// thread, P2 = A / R1 threads per transform.
+__host__ __device__ constexpr int team_log2r1(int log2A) { return log2A < 4 ? log2A : 4; }
+// Row stride of the staged band in LDS: A + P2 keeps the reads of one
+// instruction on distinct banks.
+__host__ __device__ constexpr int team_zs_stride(int log2A) { return (1 << log2A) + (1 << (log2A - team_log2r1(log2A))); }
+
 // Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A.
-// zs holds the row's Z[qb0 + ql + B qf] at [ql * A + qf].
+//   zs  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
+//   pre [k][l']       = W_F^{P2 k l'}             (input factor)
+//   post[k1][l'][j]   = W_F^{j (l' + (F/A) k1)}  (between the register passes, P2 > 1)
+// Result U[f][qb0 .. qb0+Qb), stored through an LDS tile in Qb-element runs.
 // The passes are separate (not inlined) functions: inlined into the row loop
 // together they exceed the register file and spill.
 template <int LOG2A>
 __device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict__ U, const float2* zs,
-                                        const float2* wf, const float2* wqf, float2* buf) {
+                                        const float2* pre, const float2* post, float2* buf) {
   constexpr int A = 1 << LOG2A;
-  constexpr int R1 = A < 16 ? A : 16;
-  constexpr int P2 = A / R1;
-  static_assert(P2 <= R1, "A too large for two passes");
-  const int F = 1 << log2F;
+  constexpr int LOG2R1 = team_log2r1(LOG2A);
+  constexpr int R1 = 1 << LOG2R1;
+  constexpr int LOG2P2 = LOG2A - LOG2R1;
+  constexpr int P2 = 1 << LOG2P2;
+  constexpr int ZS = team_zs_stride(LOG2A);
   const int log2Ph = log2F - LOG2A;  // phases l' per qb
   const int log2Qb = 12 - log2F;
-  const int ntr = (1 << log2Qb) << log2Ph;  // transforms
-  const int rounds = (ntr * P2 + kTeamThreads - 1) / kTeamThreads;
-  for (int rd = 0; rd < rounds; ++rd) {
-    const int e = rd * kTeamThreads + threadIdx.x;
-    const int tr = e / P2, j = e % P2;
-    const bool active = tr < ntr;
+  const int TS = (1 << log2Qb) + 1;  // padded tile row
+  if constexpr (P2 == 1) {
+    // 4096/A transforms of A <= 16 points: 16/A rounds of 256
+#pragma unroll 1
+    for (int rd = 0; rd < 16 / A; ++rd) {
+      const int tr = rd * kTeamThreads + threadIdx.x;
+      const int ql = tr >> log2Ph, lp = tr & ((1 << log2Ph) - 1);
+      float2 v[R1];
+#pragma unroll
+      for (int k = 0; k < R1; ++k) v[k] = cmul(zs[ql * ZS + k], pre[(k << log2Ph) + lp]);
+      dft_reg<R1, 1>(v);
+#pragma unroll
+      for (int m = 0; m < R1; ++m) buf[(lp + (m << log2Ph)) * TS + ql] = v[m];
+    }
+  } else {
+    // 4096/A transforms of A = 16 P2 points, P2 threads each: one round
+    static_assert(R1 == 16 && P2 <= 16, "A out of range");
+    constexpr int XS = A + P2;  // exchange [tr][j][k1], j stride R1 + 1
+    const int tr = threadIdx.x >> LOG2P2, j = threadIdx.x & (P2 - 1);
     const int ql = tr >> log2Ph, lp = tr & ((1 << log2Ph) - 1);
     float2 v[R1];
-    if (active) {
 #pragma unroll
-      for (int k = 0; k < R1; ++k) {
-        const int qf = j + P2 * k;
-        v[k] = cmul(zs[ql * A + qf], wf[(qf * lp) & (F - 1)]);
-      }
-      dft_reg<R1, 1>(v);
+    for (int k = 0; k < R1; ++k) v[k] = cmul(zs[ql * ZS + j + P2 * k], pre[(k << log2Ph) + lp]);
+    dft_reg<R1, 1>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], post[(((k1 << log2Ph) + lp) << LOG2P2) + j]);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[tr * XS + j * (R1 + 1) + k1] = v[k1];
+    __syncthreads();
+    constexpr int Uu = R1 / P2;
+#pragma unroll
+    for (int u = 0; u < Uu; ++u) {
+      const int k1 = u * P2 + j;
+      float2 w2[P2];
+#pragma unroll
+      for (int jj = 0; jj < P2; ++jj) w2[jj] = buf[tr * XS + jj * (R1 + 1) + k1];
+      dft_reg<P2, 1>(w2);
+#pragma unroll
+      for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w2[k2];
     }
-    if constexpr (P2 > 1) {
-      // inter-pass twiddle W_A^{j k1} = W_F^{j k1 F/A}, exchange, P2-point DFTs
-      constexpr int XS = R1 * (P2 + 1);
-      if (active) {
+    __syncthreads();
+    // output m' = k1 + R1 k2 -> f = l' + (F/A) m', into the tile [f][ql]
 #pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], wf[(j * k1) << log2Ph]);
+    for (int u = 0; u < Uu; ++u)
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) buf[tr * XS + k1 * (P2 + 1) + j] = v[k1];
+      for (int k2 = 0; k2 < P2; ++k2) {
+        const int f = lp + ((u * P2 + j + R1 * k2) << log2Ph);
+        buf[f * TS + ql] = v[u * P2 + k2];
       }
-      __syncthreads();
-      constexpr int Uu = R1 / P2;
-      if (active) {
-#pragma unroll
-        for (int u = 0; u < Uu; ++u) {
-          const int k1 = j * Uu + u;
-          float2 w2[P2];
-#pragma unroll
-          for (int jj = 0; jj < P2; ++jj) w2[jj] = buf[tr * XS + k1 * (P2 + 1) + jj];
-          dft_reg<P2, 1>(w2);
-#pragma unroll
-          for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w2[k2];
-        }
-      }
-      __syncthreads();
-      // results to the LDS tile [f][ql] (output m = k1 + R1 k2, f = lp + m F/A)
-      if (active) {
-#pragma unroll
-        for (int u = 0; u < Uu; ++u)
-#pragma unroll
-          for (int k2 = 0; k2 < P2; ++k2) {
-            const int f = lp + (((j * Uu + u) + R1 * k2) << log2Ph);
-            buf[(f << log2Qb) + ql] = cmul(v[u * P2 + k2], wqf[ql * F + f]);
-          }
-      }
-    } else {
-      if (active) {
-#pragma unroll
-        for (int m = 0; m < R1; ++m) {
-          const int f = lp + (m << log2Ph);
-          buf[(f << log2Qb) + ql] = cmul(v[m], wqf[ql * F + f]);
-        }
-      }
-    }
   }
   __syncthreads();
-  // coalesced stores of U[f][qb0 .. qb0 + Qb)
   for (int e = threadIdx.x; e < kTeamPoints; e += kTeamThreads) {
     const int f = e >> log2Qb, ql = e & ((1 << log2Qb) - 1);
-    gstore(U + f * kTeamB + qb0 + ql, buf[e]);
+    gstore(U + f * kTeamB + qb0 + ql, buf[f * TS + ql]);
   }
 }
 
-// Pass 2 for one row: 16 consecutive f starting at f0, 256-point inverse FFTs
-// over qb, written to out (the row) scaled by norm.
+// Pass 2 for one row: 16 consecutive f starting at f0, the factor W_N^{qb f}
+// and 256-point inverse FFTs over qb = j + 16 k, written to out (the row)
+// scaled by norm.
+//   t2[g][k]  = W_N^{16 k (f0 + g)}   wb[k1][j] = W_B^{j k1}
+//   base      = W_N^{j (f0 + g)} for this thread
 __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, float norm, int log2F, int f0,
-                                        const gfloat2* U, const float2* wb, float2* buf) {
+                                        const gfloat2* U, const float2* t2, const float2* wb, float2 base,
+                                        float2* buf) {
   constexpr int R1 = 16, P2 = 16, XS = R1 * (P2 + 1), OS = 17;
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   float2 v[R1];
 #pragma unroll
   for (int k = 0; k < R1; ++k) v[k] = ld_agent2(U + (size_t)(f0 + g) * kTeamB + j + P2 * k);
-  dft_reg<R1, 1>(v);
 #pragma unroll
-  for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], wb[j * k1]);
+  for (int k = 1; k < R1; ++k) v[k] = cmul(v[k], t2[g * R1 + k]);
+  dft_reg<R1, 1>(v);
+  v[0] = cmul(v[0], base);
+#pragma unroll
+  for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wb[k1 * P2 + j]));
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) buf[g * XS + k1 * (P2 + 1) + j] = v[k1];
   __syncthreads();
@@ -210,16 +226,15 @@ __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, float 
 }
 
 __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
-  constexpr int BUF = kTeamB * 17;  // pass-2 exchange / transpose
-  // pass-1 exchange: (4096/A) transforms x 16 * (A/16 + 1) entries, largest at A = 32
-  constexpr int BUF1 = kTeamPoints + kTeamPoints / 2;
-  constexpr int BUFMAX = BUF > BUF1 ? BUF : BUF1;
+  // buf: pass-1 exchange ((256 / P2) (A + P2) = 4352) and tile
+  // (F (Qb + 1) <= 4608); pass-2 exchange / transpose (4352)
+  constexpr int BUFMAX = kTeamPoints + (1 << kTeamMaxLog2F);
   __shared__ float2 buf[BUFMAX];
-  __shared__ float2 wf[1 << kTeamMaxLog2F];  // W_F^r
-  __shared__ float2 wqf[kTeamPoints];        // W_N^{qb f} for the member's qb
-  __shared__ float2 xs[kTeamPoints / 2];     // X[qb + B qf], qf < F/2
-  __shared__ float2 zs[kTeamPoints / 2];     // the row's Z on the member's bins
-  __shared__ float2 wb[kTeamB];              // W_B^r
+  __shared__ float2 zs[kTeamPoints / 2 + kTeamPoints / 32];  // the row's Z on the member's bins
+  __shared__ float2 pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
+  __shared__ float2 post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
+  __shared__ float2 t2s[kTeamB];             // pass-2 [g][k]
+  __shared__ float2 wb[kTeamB];              // pass-2 [k1][j]
   __shared__ SegD sseg[kTeamSegLds];
   __shared__ int s_rank, s_flag;
   // plain copies of the arguments (never take the address of the kernel argument)
@@ -259,32 +274,48 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
   const int log2Qb = 12 - log2F;
   const int items = F >> 4;  // pass-1 qb blocks == pass-2 f blocks per row
   // Each member owns exactly one pass-1 qb block and one pass-2 f block
-  // (items == team_size for F <= 512), so its tables are filled once.
+  // (items == team_size for F <= 512), so the pass-2 tables are filled once.
   const int item = member;
   const bool has_item = item < items;
   const int qb0 = item << log2Qb;
-  for (int e = threadIdx.x; e < F; e += kTeamThreads) wf[e] = tw((unsigned)e << kTeamLog2B);
-  for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) wb[e] = tw((unsigned)e << log2F);
-  if (has_item) {
-    for (int e = threadIdx.x; e < kTeamPoints; e += kTeamThreads) {
-      const int ql = e >> log2F, f = e & (F - 1);
-      wqf[e] = tw((unsigned)((qb0 + ql) * f));
-    }
-    for (int e = threadIdx.x; e < kTeamPoints / 2; e += kTeamThreads) {
-      const int ql = e >> (log2F - 1), qf = e & ((F >> 1) - 1);
-      xs[e] = spec[qb0 + ql + (qf << kTeamLog2B)];
-    }
+  const int f0 = item << 4;
+  for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) {
+    const int a = e >> 4, b = e & 15;  // t2s[g][k] with g = a, k = b; wb[k1][j] with k1 = a, j = b
+    t2s[e] = tw((unsigned)((16 * b * (f0 + a)) & (N - 1)));
+    wb[e] = tw((unsigned)(a * b) << log2F);
   }
+  const float2 base = tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4))));
   const int nrows = ta.team_count[team];
+  const int nbuf = ta.nbuf, lag = nbuf - 1;
   float2* const scratch = ta.scratch + (size_t)team * kTeamBuffers * N;
   unsigned* const arrive = sync + 64 * (1 + team * kTeamBuffers);
-  for (int it = 0; it <= nrows; ++it) {
+  int tab_log2A = -1;  // pass-1 tables currently built for this A
+  // Iteration it runs pass 1 of row it and pass 2 of row it - lag: with
+  // nbuf buffers a row's pass 2 comes nbuf - 1 iterations after its pass 1.
+  for (int it = 0; it < nrows + lag; ++it) {
     if (it < nrows) {
       const RowD r = rows[team_rows[team + nteams * it]];
-      const int b = it % kTeamBuffers, gen = it / kTeamBuffers;
-      __syncthreads();  // previous users of sseg and buf are done
+      const int b = it % nbuf, gen = it / nbuf;
+      __syncthreads();  // previous users of sseg, zs, the tables and buf are done
       const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
       if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = segs[r.seg_begin + threadIdx.x];
+      const int log2A = r.log2A, A = 1 << log2A;
+      if (log2A != tab_log2A) {
+        tab_log2A = log2A;
+        const int log2R1 = team_log2r1(log2A), R1 = 1 << log2R1;
+        const int log2P2 = log2A - log2R1, P2 = 1 << log2P2;
+        const int log2Ph = log2F - log2A;
+        for (int e = threadIdx.x; e < (R1 << log2Ph); e += kTeamThreads) {
+          // [k][l'] = W_F^{P2 k l'} = W_N^{B P2 k l'}; P2 k l' < F
+          const int lp = e & ((1 << log2Ph) - 1), k = e >> log2Ph;
+          pre[e] = tw((unsigned)(k * lp) << (log2P2 + kTeamLog2B));
+        }
+        if (P2 > 1)
+          for (int e = threadIdx.x; e < F; e += kTeamThreads) {  // [k1][l'][j] = W_F^{j (l' + (F/A) k1)}
+            const int j = e & (P2 - 1), lp = (e >> log2P2) & ((1 << log2Ph) - 1), k1 = e >> (log2P2 + log2Ph);
+            post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << kTeamLog2B);
+          }
+      }
       if (gen > 0) {
         if (!team_wait(arrive + 64 * b + 32, (unsigned)(team_size * gen), &s_flag, error)) return;
       } else {
@@ -293,24 +324,25 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
       gfloat2* U = (gfloat2*)(scratch + (size_t)b * N);
       if (has_item) {
         // the row's windowed band on this member's bins, into LDS
-        const int log2A = r.log2A, A = 1 << log2A, half = F >> 1;
         const float maxidx = (float)N - 1.0f;
-        for (int e = threadIdx.x; e < (1 << log2Qb) * A; e += kTeamThreads) {
-          const int ql = e >> log2A, qf = e & (A - 1);
-          const float w = window_at(qb0 + ql + (qf << kTeamLog2B), nseg, sseg, mother, maxidx);
-          const float2 x = xs[ql * half + qf];
-          zs[e] = make_float2(x.x * w, x.y * w);
+        const int zstride = team_zs_stride(log2A);
+        for (int e = threadIdx.x; e < (A << log2Qb); e += kTeamThreads) {
+          const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
+          const int q = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
+          const float w = window_at(q, nseg, sseg, mother, maxidx);
+          const float2 x = spec[q];
+          zs[ql * zstride + qf] = make_float2(x.x * w, x.y * w);
         }
         __syncthreads();
         switch (log2A) {
-          case 1: team_pass1<1>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 2: team_pass1<2>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 3: team_pass1<3>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 4: team_pass1<4>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 5: team_pass1<5>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 6: team_pass1<6>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          case 7: team_pass1<7>(log2F, qb0, U, zs, wf, wqf, buf); break;
-          default: team_pass1<8>(log2F, qb0, U, zs, wf, wqf, buf); break;
+          case 1: team_pass1<1>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 2: team_pass1<2>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 3: team_pass1<3>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 4: team_pass1<4>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 5: team_pass1<5>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 6: team_pass1<6>(log2F, qb0, U, zs, pre, post, buf); break;
+          case 7: team_pass1<7>(log2F, qb0, U, zs, pre, post, buf); break;
+          default: team_pass1<8>(log2F, qb0, U, zs, pre, post, buf); break;
         }
       }
       // the pass-1 stores are complete (s_waitcnt) before the arrival is counted
@@ -318,14 +350,14 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
       __syncthreads();
       if (threadIdx.x == 0) add_agent(arrive + 64 * b, 1u);
     }
-    if (it >= 1) {
-      const int prev = it - 1;
+    if (it >= lag) {
+      const int prev = it - lag;
       const RowD r = rows[team_rows[team + nteams * prev]];
-      const int b = prev % kTeamBuffers, gen = prev / kTeamBuffers;
+      const int b = prev % nbuf, gen = prev / nbuf;
       if (!team_wait(arrive + 64 * b, (unsigned)(team_size * (gen + 1)), &s_flag, error)) return;
       if (has_item)
-        team_pass2((gfloat2*)(out + r.out_off), n, norm, log2F, item << 4, (const gfloat2*)(scratch + (size_t)b * N),
-                   wb, buf);
+        team_pass2((gfloat2*)(out + r.out_off), n, norm, log2F, f0, (const gfloat2*)(scratch + (size_t)b * N), t2s, wb,
+                   base, buf);
       // every read of this buffer has landed in registers (they fed the LDS
       // exchange), so it may be reused; the output stores need not complete
       __syncthreads();
