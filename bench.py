@@ -7,14 +7,28 @@ output 3000 x 100,000 complex64 (2.4 GB) written to HBM.
 
 A step is one full cwt (forward FFT + every scale) on device pointers: the
 input is already in HBM and the output buffer preallocated there.  The value
-is output GB/s = 8 * fn * n bytes / step time.  The roofline entry reports the
-same algorithmic bytes over the summed kernel time of the step, from HIP
-events recorded on the launch stream, against MI355X's 8 TB/s HBM peak.
-The CPU baseline is the oracle's float32 port (numpy float32 FFTs plus the
-reference's window multiply in C) on one host core, timed on a bounded
-sample of the same workload (the first `--cpu-rows` scales).
+is output GB/s = 8 * fn * n bytes / step time (HIP events around the K steps).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R]
+The roofline entry is for the dominant kernel (the one with the most time):
+its algorithmic bytes per launch (8 B per complex64 output value it writes,
+i.e. 8 * n * rows it owns; 4 n + 4 N for the forward FFT) over its average
+launch time from HIP events recorded around each launch on the launch stream,
+against MI355X's 8 TB/s HBM peak.  `traffic` is the HBM bytes per launch of
+that kernel from rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled
+by what the counter reads for a device copy of known size; it is read from
+the summary that `--summarize-profiles` wrote under profiles/ (null if absent).
+
+The CPU baseline is the oracle's float32 port (numpy float32 FFTs plus the
+reference's window multiply in C) on one host core over the first
+`--cpu-rows` scales of the same workload (all 3000 by default).
+
+Usage:
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R] [--no-cpu]
+  python bench.py --copy-probe MIB     one device copy of MIB MiB (counter calibration)
+  python bench.py --summarize-profiles DIR
+        write profiles/ summaries from the rocprofv3 CSV output under DIR:
+        trace/ (--kernel-trace --stats) and pmc_fetch/, pmc_write/,
+        copy_fetch/, copy_write/ (--pmc runs)
 """
 from __future__ import annotations
 
@@ -51,17 +65,176 @@ def cpu_baseline(x, scales, fb, rows):
     return 8.0 * rows * x.size / dt / 1e9, dt
 
 
+# rocprofv3 kernel names of each kernel kind the plan profiles
+KIND_KERNELS = {
+    "forward_fft": ("k_fwd_small", "k_fwd_a", "k_fwd_b", "k_fwd_post"),
+    "row_writer_direct": ("k_rows_direct",),
+    "row_team": ("k_rows_team",),
+    "row_spreader": ("k_spreader",),
+    "row_writer_two_pass": ("k_writer",),
+}
+WORKLOAD_TAG = "100k_x_3000"
+
+
+def kind_bytes(kind, n, N, split):
+    """Algorithmic HBM bytes one launch of `kind` must move in this workload."""
+    if kind == "forward_fft":
+        return 4.0 * n + 4.0 * N  # signal read + N/2 complex64 spectrum written
+    rows = {"row_writer_direct": split["direct"], "row_team": split["team"],
+            "row_writer_two_pass": split["two_pass"]}.get(kind, 0)
+    return 8.0 * n * rows
+
+
+def traffic_summary(kind):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"{kind}_{WORKLOAD_TAG}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    nbytes = int(mib) << 20
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes}))
+
+
+def _pmc_per_dispatch(csv_path, counter, names):
+    """Mean over dispatches of kernels whose name contains one of `names` of
+    the counter's value (summed over the rows of one dispatch)."""
+    import csv
+
+    per = {}
+    with open(csv_path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter or not any(k in r["Kernel_Name"] for k in names):
+                continue
+            key = r["Dispatch_Id"]
+            per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    if not per:
+        raise SystemExit(f"{csv_path}: no {counter} rows for {names}")
+    return sum(per.values()) / len(per), len(per)
+
+
+TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv "
+             "-d $GRAFT_REPO_ROOT/gpurun_out/prof/trace -o run -- python3 $GRAFT_REPO_ROOT/bench.py --steps 5 "
+             "--warmup 2 --no-cpu")
+
+
+def summarize_trace(root):
+    """profiles/cwt_100k_x_3000_kernel_trace.md from rocprofv3 --kernel-trace
+    --stats output in root/trace/run_kernel_stats.csv."""
+    import csv
+
+    path = os.path.join(root, "trace", "run_kernel_stats.csv")
+    with open(path) as f:
+        rows = list(csv.DictReader(f))
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                       f"cwt_{WORKLOAD_TAG}_kernel_trace.md")
+    with open(out, "w") as f:
+        f.write("# Kernel times, cwt 100k x 3000 (rocprofv3 --kernel-trace --stats)\n\n")
+        f.write(f"Command (on the GPU box):\n\n    {TRACE_CMD}\n\n")
+        f.write("Summarized by `python bench.py --summarize-profiles gpurun_out/prof`.\n"
+                "Calls include the 2 warm-up steps and 5 timed steps of bench.py.\n\n")
+        f.write("| kernel | calls | average (us) | min (us) | max (us) | share (%) |\n|---|---|---|---|---|---|\n")
+        for r in rows:
+            f.write(f"| `{r['Name'][:80]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                    f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |\n")
+    print(f"wrote {out}")
+
+
+def summarize_profiles(root):
+    if os.path.exists(os.path.join(root, "trace", "run_kernel_stats.csv")):
+        summarize_trace(root)
+    if os.path.exists(os.path.join(root, "pmc_fetch", "run_counter_collection.csv")):
+        summarize_pmc(root)
+
+
+def summarize_pmc(root):
+    """Write profiles/<kind>_100k_x_3000_traffic.{json,md} from four rocprofv3
+    --pmc runs under `root`: pmc_fetch/ and pmc_write/ (bench.py --steps 2
+    --warmup 1 --no-cpu) and copy_fetch/ and copy_write/ (bench.py
+    --copy-probe 1024), each holding run_counter_collection.csv."""
+    kind = "row_team"
+    names = KIND_KERNELS[kind]
+    copy_bytes = 1024 << 20
+    cf, _ = _pmc_per_dispatch(os.path.join(root, "copy_fetch", "run_counter_collection.csv"), "FETCH_SIZE",
+                              ("copyBuffer", "copy", "elementwise"))
+    cw, _ = _pmc_per_dispatch(os.path.join(root, "copy_write", "run_counter_collection.csv"), "WRITE_SIZE",
+                              ("copyBuffer", "copy", "elementwise"))
+    kf, nd = _pmc_per_dispatch(os.path.join(root, "pmc_fetch", "run_counter_collection.csv"), "FETCH_SIZE", names)
+    kw, _ = _pmc_per_dispatch(os.path.join(root, "pmc_write", "run_counter_collection.csv"), "WRITE_SIZE", names)
+    f_fetch = copy_bytes / (cf * 1024.0)
+    f_write = copy_bytes / (cw * 1024.0)
+    fetch_b = kf * 1024.0 * f_fetch
+    write_b = kw * 1024.0 * f_write
+    cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
+           "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py {a}")
+    summary = {
+        "kernel": "fcwt_hip::k_rows_team", "kind": kind, "workload": WORKLOAD_TAG,
+        "dispatches": nd,
+        "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw,
+        "copy_probe_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
+        "fetch_scale": f_fetch, "write_scale": f_write,
+        "fetch_bytes_per_launch": fetch_b, "write_bytes_per_launch": write_b,
+        "traffic_bytes_per_launch": fetch_b + write_b,
+        "commands": [cmd.format(c="FETCH_SIZE", d="pmc_fetch", a="--steps 2 --warmup 1 --no-cpu"),
+                     cmd.format(c="WRITE_SIZE", d="pmc_write", a="--steps 2 --warmup 1 --no-cpu"),
+                     cmd.format(c="FETCH_SIZE", d="copy_fetch", a="--copy-probe 1024"),
+                     cmd.format(c="WRITE_SIZE", d="copy_write", a="--copy-probe 1024")],
+        "summarized_by": "python bench.py --summarize-profiles gpurun_out/prof",
+    }
+    out_dir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+    os.makedirs(out_dir, exist_ok=True)
+    base = os.path.join(out_dir, f"{kind}_{WORKLOAD_TAG}_traffic")
+    with open(base + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    with open(base + ".md", "w") as f:
+        f.write(f"# HBM traffic of k_rows_team, cwt 100k x 3000 (rocprofv3 --pmc)\n\n")
+        f.write("Commands (one counter per run, on the GPU box):\n\n")
+        for c in summary["commands"]:
+            f.write(f"    {c}\n")
+        f.write(f"\nSummarized by `{summary['summarized_by']}`.\n\n")
+        f.write("| quantity | value |\n|---|---|\n")
+        f.write(f"| k_rows_team dispatches | {nd} |\n")
+        f.write(f"| FETCH_SIZE per dispatch | {kf:.0f} KiB |\n")
+        f.write(f"| WRITE_SIZE per dispatch | {kw:.0f} KiB |\n")
+        f.write(f"| copy probe (1 GiB) FETCH_SIZE / WRITE_SIZE | {cf:.0f} / {cw:.0f} KiB |\n")
+        f.write(f"| scale factors fetch / write | {f_fetch:.4f} / {f_write:.4f} |\n")
+        f.write(f"| HBM read bytes per launch | {fetch_b / 1e9:.4f} GB |\n")
+        f.write(f"| HBM write bytes per launch | {write_b / 1e9:.4f} GB |\n")
+        f.write(f"| traffic per launch | {(fetch_b + write_b) / 1e9:.4f} GB |\n")
+    print(json.dumps(summary))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-rows", type=int, default=1000)
+    ap.add_argument("--cpu-rows", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
+    ap.add_argument("--summarize-profiles", default="", metavar="DIR")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: FCWT::cwt runs on a single MI355X here "
                  "(multi-GPU sharding is described in DESIGN.md, not run); use --gpus 1")
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles)
+        return
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import torch
 
@@ -73,6 +246,7 @@ def main():
     morl = fcwt_amd.Morlet(fb)
     scs = fcwt_amd.Scales(morl, fcwt_amd.FCWT_LOGSCALES, fs, f0, f1, fn)
     plan = fcwt_amd.Plan(n, scs.scales, fb, threads=1, normalize=True)
+    split = plan.row_split()
     dev = torch.device("cuda:0")
     dx = torch.from_numpy(x).to(dev)
     out = torch.empty((fn, n), dtype=torch.complex64, device=dev)
@@ -84,6 +258,7 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    plan.check()
     plan.profile(True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -96,13 +271,18 @@ def main():
     ms_step = e0.elapsed_time(e1) / args.steps
     prof = plan.profile_read()
     plan.profile(False)
+    plan.check()
 
     out_bytes = 8.0 * fn * n
-    in_bytes = 4.0 * n
     value = out_bytes / (ms_step * 1e-3) / 1e9
-    kernel_ms = sum(ms for ms, _ in prof.values()) / args.steps
-    achieved = (out_bytes + in_bytes) / (kernel_ms * 1e-3) / 1e9
-    kernels = {k: {"ms_per_step": ms / args.steps, "launches_per_step": c / args.steps} for k, (ms, c) in prof.items()}
+    kernels = {}
+    for k, (ms, c) in prof.items():
+        b = kind_bytes(k, n, plan.N, split)
+        kernels[k] = {"ms_per_launch": ms / c, "launches_per_step": c / args.steps,
+                      "bytes_per_launch": b, "gbs": b / (ms / c * 1e-3) / 1e9}
+    dom = max(prof, key=lambda k: prof[k][0])
+    d = kernels[dom]
+    tr = traffic_summary(dom)
 
     result = {
         "metric": METRIC,
@@ -119,11 +299,17 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cwt 100k x 3000: n=100000 (N=131072) float32, 5 tones + N(0,0.2) noise, fs=1000, "
                                "Morlet sigma=2, 3000 LOGSCALES 1-450 Hz, complex64 output 2.4 GB",
-                   "n": n, "fn": fn, "N": plan.N, "samples_scales_per_s": round(fn * n / (ms_step * 1e-3), 1),
+                   "n": n, "fn": fn, "N": plan.N, "rows": split,
+                   "samples_scales_per_s": round(fn * n / (ms_step * 1e-3), 1),
                    "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "whole cwt step (all kernels, summed event time)", "kernels": kernels},
+        "roofline": {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": round(tr["traffic_bytes_per_launch"]) if tr else None,
+                     "kernel": f"{dom} ({', '.join(KIND_KERNELS.get(dom, (dom,)))}): "
+                               f"{d['bytes_per_launch'] / 1e9:.4f} GB algorithmic per launch, "
+                               f"{d['ms_per_launch']:.4f} ms per launch",
+                     "traffic_source": f"profiles/{dom}_{WORKLOAD_TAG}_traffic.json" if tr else None,
+                     "kernels": kernels},
     }
     if not args.no_cpu:
         rows = min(args.cpu_rows, fn)

@@ -48,6 +48,7 @@ SIGNATURES = {
     "fcwt_hip_plan_profile_read": (ctypes.c_int, [_plan_t, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
                                                   ctypes.POINTER(ctypes.c_double),
                                                   ctypes.POINTER(ctypes.c_longlong), _c_int_p]),
+    "fcwt_hip_plan_row_split": (ctypes.c_int, [_plan_t, _c_int_p, _c_int_p, _c_int_p]),
     "fcwt_hip_cwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_cwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }

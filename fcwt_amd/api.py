@@ -164,6 +164,13 @@ class Plan:
         nat.check(nat.lib().fcwt_hip_plan_device_bytes(self._h, ctypes.byref(b)), "plan_device_bytes")
         return b.value
 
+    def row_split(self) -> dict:
+        """Rows per row kernel: {"direct": d, "team": t, "two_pass": w}."""
+        d, t, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.lib().fcwt_hip_plan_row_split(self._h, ctypes.byref(d), ctypes.byref(t), ctypes.byref(w)),
+                  "plan_row_split")
+        return {"direct": d.value, "team": t.value, "two_pass": w.value}
+
     def check(self) -> None:
         """Wait for the device; raise if an in-kernel synchronisation failed."""
         nat.check(nat.lib().fcwt_hip_plan_check(self._h), "fcwt_hip_plan_check")

@@ -334,10 +334,11 @@ struct fcwt_hip_plan_s {
   std::vector<Item> items;  // direct writer items, then per batch: spreader items, writer items
   // XCD-team path for wide bands (rows_team.h)
   bool team_capable = false;
-  int team_size = 0, teams_per_xcd = 0, nteams = 0, team_log2F = 0, team_nbuf = kTeamBuffers;
-  std::vector<int> team_rows, team_count;
+  int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, seq_stride = 0;
+  std::vector<int> team_rows, team_seq, team_blocks;
   int* d_team_rows = nullptr;
-  int* d_team_count = nullptr;
+  int* d_team_seq = nullptr;
+  int* d_team_blocks = nullptr;
   float2* d_team_scratch = nullptr;
   unsigned* d_team_sync = nullptr;
   unsigned* d_error = nullptr;
@@ -437,7 +438,7 @@ static void free_plan(fcwt_hip_plan p) {
   for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
-                  p->d_team_rows, p->d_team_count, p->d_team_scratch, p->d_team_sync, p->d_error};
+                  p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -497,23 +498,41 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     r.out_off = (long long)i * p->n;
   }
   if (!team.empty()) {
-    int items = 1 << (team_log2F - 4);
-    p->team_size = std::min(32, items);
-    p->teams_per_xcd = kTeamWgPerXcd / p->team_size;
-    // scratch buffers per team (pipeline depth); FCWT_HIP_TEAM_BUFFERS overrides for tuning
+    // one team per XCD; rows dealt round-robin
+    p->team_size = kTeamWgPerXcd;
+    p->nteams = 8;
+    p->team_log2F = team_log2F;
+    // scratch buffers per team: as many N-point buffers as fit the per-team
+    // scratch budget (the XCD's L2 holds them); FCWT_HIP_TEAM_BUFFERS overrides
+    const size_t row_bytes = sizeof(float2) * (size_t)p->N;
+    p->team_nbuf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
     if (const char* e = std::getenv("FCWT_HIP_TEAM_BUFFERS")) {
       int v = std::atoi(e);
-      if (v >= 1 && v <= kTeamBuffers) p->team_nbuf = v;
+      if (v >= 2 && v <= kTeamBuffers) p->team_nbuf = v;
     }
-    p->nteams = 8 * p->teams_per_xcd;
-    p->team_log2F = team_log2F;
-    int per = ((int)team.size() + p->nteams - 1) / p->nteams;
-    p->team_rows.assign((size_t)per * p->nteams, 0);
-    p->team_count.assign(p->nteams, 0);
+    const int nt = p->nteams;
+    const int per = ((int)team.size() + nt - 1) / nt;
+    p->team_rows.assign((size_t)per * nt, 0);
+    std::vector<int> count(nt, 0);
     for (size_t k = 0; k < team.size(); ++k) {
-      int t = (int)(k % p->nteams), i = (int)(k / p->nteams);
-      p->team_rows[t + (size_t)p->nteams * i] = team[k];
-      p->team_count[t] = i + 1;
+      int t = (int)(k % nt), i = (int)(k / nt);
+      p->team_rows[t + (size_t)nt * i] = team[k];
+      count[t] = i + 1;
+    }
+    // item-block sequence per team: step s = pass 1 of row s, then pass 2 of
+    // row s - lag; a pass 1 into buffer b waits for the pass 2 of the row
+    // nbuf earlier, which sits nbuf - lag steps back
+    const int lag = std::max(1, p->team_nbuf / 2);
+    p->seq_stride = 2 * per;
+    p->team_seq.assign((size_t)p->seq_stride * nt, 0);
+    p->team_blocks.assign(nt, 0);
+    for (int t = 0; t < nt; ++t) {
+      int len = 0;
+      for (int st = 0; st < count[t] + lag; ++st) {
+        if (st < count[t]) p->team_seq[(size_t)t * p->seq_stride + len++] = 2 * st;
+        if (st >= lag) p->team_seq[(size_t)t * p->seq_stride + len++] = 2 * (st - lag) + 1;
+      }
+      p->team_blocks[t] = len;
     }
   }
   // direct items, grouped by band size (one kernel instance per size)
@@ -570,8 +589,8 @@ static int check_device(int* dev) {
   return FCWT_HIP_OK;
 }
 
-// The team path launches one work-group per CU cooperatively and forms teams
-// of the 32 work-groups each XCD receives: it needs the full 256-CU device in
+// The team path launches kTeamWgPerCu work-groups per CU cooperatively and
+// forms teams from the work-groups each XCD receives: it needs the full 256-CU device in
 // one partition.  FCWT_HIP_NO_TEAMS=1 in the environment turns it off.
 static bool team_capable(int dev) {
   const char* env = std::getenv("FCWT_HIP_NO_TEAMS");
@@ -670,10 +689,11 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStrea
     TeamArgs ta;
     ta.k = a;
     ta.team_rows = p->d_team_rows;
-    ta.team_count = p->d_team_count;
+    ta.team_seq = p->d_team_seq;
+    ta.team_blocks = p->d_team_blocks;
+    ta.seq_stride = p->seq_stride;
     ta.nteams = p->nteams;
     ta.team_size = p->team_size;
-    ta.teams_per_xcd = p->teams_per_xcd;
     ta.log2F = p->team_log2F;
     ta.nbuf = p->team_nbuf;
     ta.scratch = p->d_team_scratch;
@@ -818,8 +838,9 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(dalloc(p, &p->d_error, 1));
   if (p->nteams) {
     TRY(dalloc(p, &p->d_team_rows, p->team_rows.size()));
-    TRY(dalloc(p, &p->d_team_count, p->team_count.size()));
-    TRY(dalloc(p, &p->d_team_scratch, (size_t)p->nteams * kTeamBuffers * N));
+    TRY(dalloc(p, &p->d_team_seq, p->team_seq.size()));
+    TRY(dalloc(p, &p->d_team_blocks, p->team_blocks.size()));
+    TRY(dalloc(p, &p->d_team_scratch, (size_t)p->nteams * p->team_nbuf * N));
     p->team_sync_words = (size_t)64 * (1 + p->nteams * kTeamBuffers);
     TRY(dalloc(p, &p->d_team_sync, p->team_sync_words));
   }
@@ -834,7 +855,8 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(up(p->d_items, p->items.data(), p->items.size() * sizeof(Item)));
   if (p->nteams) {
     TRY(up(p->d_team_rows, p->team_rows.data(), p->team_rows.size() * sizeof(int)));
-    TRY(up(p->d_team_count, p->team_count.data(), p->team_count.size() * sizeof(int)));
+    TRY(up(p->d_team_seq, p->team_seq.data(), p->team_seq.size() * sizeof(int)));
+    TRY(up(p->d_team_blocks, p->team_blocks.data(), p->team_blocks.size() * sizeof(int)));
   }
   k_fill_u32<<<1, 64, 0, p->stream>>>(p->d_error, 1, 0u);
 
@@ -872,7 +894,7 @@ static int read_error_word(fcwt_hip_plan p) {
   if (h != 0) {
     unsigned zero = 0;
     HIP_TRY(hipMemcpy(p->d_error, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
-    return fail(FCWT_HIP_EDEVICE, h & 2u ? "wide-band kernel: work-groups were not spread 32 per XCD"
+    return fail(FCWT_HIP_EDEVICE, h & 2u ? "wide-band kernel: work-groups were not spread evenly over the XCDs"
                                          : "wide-band kernel: a team wait timed out");
   }
   return FCWT_HIP_OK;
@@ -916,6 +938,20 @@ int fcwt_hip_plan_profile_read(fcwt_hip_plan p, int max, const char** names, dou
     ++k;
   }
   if (count) *count = k;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_plan_row_split(fcwt_hip_plan p, int* direct_rows, int* team_rows, int* two_pass_rows) {
+  if (!p) return fail(FCWT_HIP_EINVAL, "null plan");
+  int d = 0, t = 0, w = 0;
+  for (const RowD& r : p->rows) {
+    if (r.log2A == 0) ++d;
+    else if (r.log2B == kTeamLog2B && r.slot < 0) ++t;
+    else ++w;
+  }
+  if (direct_rows) *direct_rows = d;
+  if (team_rows) *team_rows = t;
+  if (two_pass_rows) *two_pass_rows = w;
   return FCWT_HIP_OK;
 }
 

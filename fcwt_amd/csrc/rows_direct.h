@@ -108,7 +108,7 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
     if (active) {
       // W_N^{j l} (constant in k1) times W_M^{j k1}
       const unsigned l = (unsigned)(l0 + g);
-      const float2 base = a.tw((unsigned)(j * l) & (unsigned)(a.N - 1));
+      const float2 base = cscale(a.tw((unsigned)(j * l) & (unsigned)(a.N - 1)), a.norm);  // with the normalisation
       v[0] = cmul(v[0], base);
 #pragma unroll
       for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[k1 * P2 + j]));
@@ -140,7 +140,7 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int k2 = 0; k2 < P2; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = cscale(v[u * P2 + k2], a.norm);
+        for (int k2 = 0; k2 < P2; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + k2];
     } else {
 #pragma unroll
       for (int m = 0; m < R1; ++m) buf[m * C::OS + g] = cscale(v[m], a.norm);
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   for (int e = tid; e < tot; e += NT) {
     int m = e / G, gg = e - m * G;
     long long t = (long long)(l0 + gg) + ((long long)m << log2L);
-    if (t < a.n) out[t] = buf[m * C::OS + gg];
+    if (t < a.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
   }
 }
 

@@ -6,20 +6,24 @@
 // With N = B * F (B = 256) and the band M = A * B:
 //   y[f + F*m] = c * sum_{qb<B} W_B^{qb m} W_N^{qb f} U[f][qb]
 //   U[f][qb]   = sum_{qf<A} Z[qb + B qf] W_F^{qf f}
-// Pass 1 (per team member: Qb = 4096/F consecutive qb): the F-point sums, done
-// as F/A phases l' of A-point inverse FFTs (f = l' + (F/A) m'), into a
-// scratch buffer U[f][qb] in global memory.  Pass 2 (per member: 16
-// consecutive f): the factor W_N^{qb f} and the B-point inverse FFTs over qb,
-// written as 16-element (128-byte) runs of the output row.
+// A pass-1 item (Qb = 4096/F consecutive qb): the F-point sums, done as F/A
+// phases l' of A-point inverse FFTs (f = l' + (F/A) m'), into a scratch
+// buffer U[f][qb] in global memory.  A pass-2 item (16 consecutive f): the
+// factor W_N^{qb f} and the B-point inverse FFTs over qb, written as
+// 16-element (128-byte) runs of the output row.  A row is F/16 items of each.
 //
-// All members of a team run on one XCD (chosen by the hardware XCC id), so
-// the scratch never leaves that XCD's L2: pass 1 stores normally, waits for
-// its stores (s_waitcnt), then bumps a per-buffer counter; pass 2 waits for
-// the counter and reads with device-scope loads, which bypass the CU's L1 and
-// are served by the shared L2.  With two scratch buffers per team, pass 1 of
-// one row overlaps pass 2 of the previous one; two work-groups per CU (two
-// teams per XCD) interleave.  Every wait is bounded: on timeout the kernel
-// records an error and every work-group exits.
+// One team per XCD: all work-groups the hardware places on that XCD (found
+// from the XCC id), so the scratch stays in the XCD's L2.  The team walks a
+// fixed sequence of item blocks -- step s holds pass 1 of row s and pass 2 of
+// row s - lag -- and member m takes items m, m + team_size, ...; every item
+// depends only on earlier ones, so the walk cannot deadlock while all members
+// are resident (cooperative launch).  A pass-1 item stores normally, waits
+// for its stores (s_waitcnt), then bumps its buffer's arrival counter; a
+// pass-2 item waits for the row's count and reads with device-scope loads,
+// which bypass the CU's L1 and are served by the shared L2, then bumps the
+// buffer's done counter, which the next pass 1 into that buffer waits for.
+// nbuf buffers per team give the sequence slack on both sides.  Every wait is
+// bounded: on timeout the kernel records an error and every work-group exits.
 //
 // Twiddle tables in LDS are laid out so that the lanes of one instruction
 // read consecutive or identical entries (no bank conflicts): pass-2 factors
@@ -34,10 +38,11 @@ namespace fcwt_hip {
 constexpr int kTeamLog2B = 8;                  // B = 256
 constexpr int kTeamB = 1 << kTeamLog2B;
 constexpr int kTeamThreads = 256;
-constexpr int kTeamWgPerCu = 2;                // two teams interleave on every CU
+constexpr int kTeamWgPerCu = 3;                // work-groups of the team on every CU
 constexpr int kTeamWgPerXcd = 32 * kTeamWgPerCu;
-constexpr int kTeamPoints = 4096;              // points per member per pass
-constexpr int kTeamBuffers = 2;
+constexpr int kTeamPoints = 4096;              // points per item
+constexpr int kTeamBuffers = 12;               // most scratch buffers per team
+constexpr size_t kTeamScratchPerXcd = 3u << 20;  // scratch bytes per team (default buffer count)
 constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
 constexpr int kTeamSegLds = 32;                // row segments staged in LDS
@@ -45,14 +50,15 @@ constexpr long kTeamSpin = 1L << 22;           // polls (with s_sleep) before a 
 
 struct TeamArgs {
   KArgs k;
-  const int* team_rows;    // rows of team t: team_rows[t + nteams * i], i < team_count[t]
-  const int* team_count;
-  int nteams;              // teams on the whole device
+  const int* team_rows;    // rows of team t: team_rows[t + nteams * i]
+  const int* team_seq;     // item blocks of team t: team_seq[t * seq_stride + s] = 2 i + pass (0 or 1)
+  const int* team_blocks;  // blocks in team t's sequence
+  int seq_stride;
+  int nteams;              // teams on the whole device (one per XCD)
   int team_size;           // work-groups per team
-  int teams_per_xcd;
   int log2F;
-  int nbuf;                // scratch buffers in use per team (1..kTeamBuffers)
-  float2* scratch;         // [team][kTeamBuffers][N]
+  int nbuf;                // scratch buffers per team (2..kTeamBuffers)
+  float2* scratch;         // [team][nbuf][N]
   unsigned* sync;          // zeroed before launch; see below
   unsigned* error;         // set to a nonzero code on a failed wait or team formation
 };
@@ -89,18 +95,17 @@ __device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, in
   return *s_flag != 0;
 }
 
-// Window factor mother[idx(q)] of the segment holding q (0 when none), with
-// the row's segments staged in LDS (fcwt.cpp:224-248).
-__device__ __forceinline__ float window_at(int q, int nseg, const SegD* sseg, const float* __restrict__ mother,
-                                           float maxidx) {
+// Index (int)fminf(N-1, step*q) into the mother table for the segment holding
+// q, or -1 when no segment holds it (fcwt.cpp:224-248); the row's segments
+// are staged in LDS.
+__device__ __forceinline__ int window_index(int q, int nseg, const SegD* sseg, float maxidx) {
 #pragma clang fp contract(off)
   int hit = -1;
   for (int s = 0; s < nseg; ++s)
     if (q >= sseg[s].lo && q < sseg[s].hi) hit = s;
-  if (hit < 0) return 0.0f;
+  if (hit < 0) return -1;
   float v = sseg[hit].step * (float)q;
-  int idx = (int)(v < maxidx ? v : maxidx);
-  return mother[idx];
+  return (int)(v < maxidx ? v : maxidx);
 }
 
 // Pass-1 register split of the A-point transform: R1 = min(A, 16) points per
@@ -109,6 +114,24 @@ __host__ __device__ constexpr int team_log2r1(int log2A) { return log2A < 4 ? lo
 // Row stride of the staged band in LDS: A + P2 keeps the reads of one
 // instruction on distinct banks.
 __host__ __device__ constexpr int team_zs_stride(int log2A) { return (1 << log2A) + (1 << (log2A - team_log2r1(log2A))); }
+// The staged band shares the work-group's LDS buffer with the exchanges and
+// the tile.  With P2 > 1 it sits at the start (pass 1 reads it into registers
+// and waits at a barrier before the exchange overwrites it); with P2 = 1 the
+// transforms run in several rounds that read it and write the tile, so it
+// sits after the tile of F (Qb + 1) values.
+__host__ __device__ constexpr int team_zs_offset(int log2A, int log2F) {
+  return log2A > 4 ? 0 : (1 << log2F) * ((1 << (12 - log2F)) + 1);
+}
+// LDS buffer the pass functions need (complex64 values) for F = 2^log2F:
+// tile F (Qb + 1) <= 4608, exchanges 4352, band after the tile when A <= 16
+__host__ __device__ constexpr int team_buf_need(int log2F) {
+  return team_zs_offset(4, log2F) + (1 << (12 - log2F)) * 17;
+}
+constexpr int kTeamBuf = 4768;
+static_assert(kTeamThreads == kTeamB, "one thread per pass-2 table entry");
+static_assert(team_buf_need(7) <= kTeamBuf && team_buf_need(8) <= kTeamBuf && team_buf_need(9) <= kTeamBuf &&
+                  (1 << 9) * ((1 << 3) + 1) <= kTeamBuf && 4352 <= kTeamBuf,
+              "team LDS buffer too small");
 
 // Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A.
 //   zs  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
@@ -154,6 +177,7 @@ __device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict_
     dft_reg<R1, 1>(v);
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], post[(((k1 << log2Ph) + lp) << LOG2P2) + j]);
+    __syncthreads();  // every read of the band (same LDS) is done
 #pragma unroll
     for (int k1 = 0; k1 < R1; ++k1) buf[tr * XS + j * (R1 + 1) + k1] = v[k1];
     __syncthreads();
@@ -187,10 +211,10 @@ __device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict_
 
 // Pass 2 for one row: 16 consecutive f starting at f0, the factor W_N^{qb f}
 // and 256-point inverse FFTs over qb = j + 16 k, written to out (the row)
-// scaled by norm.
+// with non-temporal stores (written once, never re-read here).
 //   t2[g][k]  = W_N^{16 k (f0 + g)}   wb[k1][j] = W_B^{j k1}
-//   base      = W_N^{j (f0 + g)} for this thread
-__device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, float norm, int log2F, int f0,
+//   base      = c W_N^{j (f0 + g)} for this thread, c the row normalisation
+__device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int log2F, int f0,
                                         const gfloat2* U, const float2* t2, const float2* wb, float2 base,
                                         float2* buf) {
   constexpr int R1 = 16, P2 = 16, XS = R1 * (P2 + 1), OS = 17;
@@ -214,23 +238,21 @@ __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, float 
   __syncthreads();
   // m = j + 16 k2; transpose to [m][g]
 #pragma unroll
-  for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = cscale(w[k2], norm);
+  for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = w[k2];
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < (kTeamB * 16) / kTeamThreads; ++s) {
     const int e = s * kTeamThreads + threadIdx.x;
     const int m = e >> 4, gg = e & 15;
     const long long t = (long long)(f0 + gg) + ((long long)m << log2F);
-    if (t < n) gstore(out + t, buf[m * OS + gg]);
+    if (t < n) gstore_nt(out + t, buf[m * OS + gg]);
   }
 }
 
 __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
-  // buf: pass-1 exchange ((256 / P2) (A + P2) = 4352) and tile
-  // (F (Qb + 1) <= 4608); pass-2 exchange / transpose (4352)
-  constexpr int BUFMAX = kTeamPoints + (1 << kTeamMaxLog2F);
-  __shared__ float2 buf[BUFMAX];
-  __shared__ float2 zs[kTeamPoints / 2 + kTeamPoints / 32];  // the row's Z on the member's bins
+  // buf: the staged band, pass-1 exchange and tile, pass-2 exchange and
+  // transpose (team_zs_offset, team_buf_need)
+  __shared__ float2 buf[kTeamBuf];
   __shared__ float2 pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
   __shared__ float2 post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
   __shared__ float2 t2s[kTeamB];             // pass-2 [g][k]
@@ -262,41 +284,37 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
   if (!team_wait(&sync[8], gridDim.x, &s_flag, error)) return;
   if (threadIdx.x == 0) {
     bool ok = true;
-    for (int x = 0; x < 8; ++x) ok &= ld_agent(&sync[x]) == (unsigned)(team_size * ta.teams_per_xcd);
+    for (int x = 0; x < 8; ++x) ok &= ld_agent(&sync[x]) == (unsigned)team_size;
     if (!ok) atomicOr(error, 2u);
     s_flag = ok;
   }
   __syncthreads();
   if (!s_flag) return;
-  const int team = xcc * ta.teams_per_xcd + s_rank / team_size;
-  const int member = s_rank % team_size;
+  const int team = xcc;
+  const int member = s_rank;
   const int F = 1 << log2F;
   const int log2Qb = 12 - log2F;
-  const int items = F >> 4;  // pass-1 qb blocks == pass-2 f blocks per row
-  // Each member owns exactly one pass-1 qb block and one pass-2 f block
-  // (items == team_size for F <= 512), so the pass-2 tables are filled once.
-  const int item = member;
-  const bool has_item = item < items;
-  const int qb0 = item << log2Qb;
-  const int f0 = item << 4;
+  const int log2I = log2F - 4;  // items per pass per row: F/16
   for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) {
-    const int a = e >> 4, b = e & 15;  // t2s[g][k] with g = a, k = b; wb[k1][j] with k1 = a, j = b
-    t2s[e] = tw((unsigned)((16 * b * (f0 + a)) & (N - 1)));
-    wb[e] = tw((unsigned)(a * b) << log2F);
+    const int k1 = e >> 4, j = e & 15;
+    wb[e] = tw((unsigned)(k1 * j) << log2F);
   }
-  const float2 base = tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4))));
-  const int nrows = ta.team_count[team];
-  const int nbuf = ta.nbuf, lag = nbuf - 1;
-  float2* const scratch = ta.scratch + (size_t)team * kTeamBuffers * N;
+  const int nbuf = ta.nbuf;
+  float2* const scratch = ta.scratch + (size_t)team * nbuf * N;
   unsigned* const arrive = sync + 64 * (1 + team * kTeamBuffers);
+  const int* __restrict__ seq = ta.team_seq + (size_t)team * ta.seq_stride;
+  const int items = ta.team_blocks[team] << log2I;
   int tab_log2A = -1;  // pass-1 tables currently built for this A
-  // Iteration it runs pass 1 of row it and pass 2 of row it - lag: with
-  // nbuf buffers a row's pass 2 comes nbuf - 1 iterations after its pass 1.
-  for (int it = 0; it < nrows + lag; ++it) {
-    if (it < nrows) {
-      const RowD r = rows[team_rows[team + nteams * it]];
-      const int b = it % nbuf, gen = it / nbuf;
-      __syncthreads();  // previous users of sseg, zs, the tables and buf are done
+  for (int it = member; it < items; it += team_size) {
+    const int sq = seq[it >> log2I], blk = it & ((1 << log2I) - 1);
+    const int i = sq >> 1;
+    const RowD r = rows[team_rows[team + nteams * i]];
+    const int b = i % nbuf, gen = i / nbuf;
+    gfloat2* const U = (gfloat2*)(scratch + (size_t)b * N);
+    __syncthreads();  // the previous item's users of LDS are done
+    if ((sq & 1) == 0) {
+      // ---- pass 1 of row i, qb block blk
+      const int qb0 = blk << log2Qb;
       const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
       if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = segs[r.seg_begin + threadIdx.x];
       const int log2A = r.log2A, A = 1 << log2A;
@@ -316,48 +334,68 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
             post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << kTeamLog2B);
           }
       }
+      // the buffer's previous row has been read by every pass-2 item
       if (gen > 0) {
-        if (!team_wait(arrive + 64 * b + 32, (unsigned)(team_size * gen), &s_flag, error)) return;
+        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error)) return;
       } else {
         __syncthreads();
       }
-      gfloat2* U = (gfloat2*)(scratch + (size_t)b * N);
-      if (has_item) {
-        // the row's windowed band on this member's bins, into LDS
-        const float maxidx = (float)N - 1.0f;
-        const int zstride = team_zs_stride(log2A);
-        for (int e = threadIdx.x; e < (A << log2Qb); e += kTeamThreads) {
-          const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
-          const int q = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
-          const float w = window_at(q, nseg, sseg, mother, maxidx);
-          const float2 x = spec[q];
-          zs[ql * zstride + qf] = make_float2(x.x * w, x.y * w);
-        }
-        __syncthreads();
-        switch (log2A) {
-          case 1: team_pass1<1>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 2: team_pass1<2>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 3: team_pass1<3>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 4: team_pass1<4>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 5: team_pass1<5>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 6: team_pass1<6>(log2F, qb0, U, zs, pre, post, buf); break;
-          case 7: team_pass1<7>(log2F, qb0, U, zs, pre, post, buf); break;
-          default: team_pass1<8>(log2F, qb0, U, zs, pre, post, buf); break;
-        }
+      // the row's windowed band on this item's bins, into LDS: at most
+      // A * Qb = 2048 values, 8 per thread, all loads issued before the
+      // first is used
+      const float maxidx = (float)N - 1.0f;
+      const int zstride = team_zs_stride(log2A);
+      float2* const zs = buf + team_zs_offset(log2A, log2F);
+      const int total = A << log2Qb;
+      constexpr int kPer = kTeamPoints / 2 / kTeamThreads;
+      int qs[kPer], idx[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = u * kTeamThreads + threadIdx.x;
+        const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
+        qs[u] = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
+        idx[u] = e < total ? window_index(qs[u], nseg, sseg, maxidx) : -1;
+      }
+      float2 xs[kPer];
+      float ws[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        xs[u] = idx[u] >= 0 ? spec[qs[u]] : make_float2(0.f, 0.f);
+        ws[u] = idx[u] >= 0 ? mother[idx[u]] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; ++u) {
+        const int e = u * kTeamThreads + threadIdx.x;
+        const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;
+        if (e < total) zs[ql * zstride + qf] = make_float2(xs[u].x * ws[u], xs[u].y * ws[u]);
+      }
+      __syncthreads();
+      switch (log2A) {
+        case 1: team_pass1<1>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 2: team_pass1<2>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 3: team_pass1<3>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 4: team_pass1<4>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 5: team_pass1<5>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 6: team_pass1<6>(log2F, qb0, U, zs, pre, post, buf); break;
+        case 7: team_pass1<7>(log2F, qb0, U, zs, pre, post, buf); break;
+        default: team_pass1<8>(log2F, qb0, U, zs, pre, post, buf); break;
       }
       // the pass-1 stores are complete (s_waitcnt) before the arrival is counted
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
       if (threadIdx.x == 0) add_agent(arrive + 64 * b, 1u);
-    }
-    if (it >= lag) {
-      const int prev = it - lag;
-      const RowD r = rows[team_rows[team + nteams * prev]];
-      const int b = prev % nbuf, gen = prev / nbuf;
-      if (!team_wait(arrive + 64 * b, (unsigned)(team_size * (gen + 1)), &s_flag, error)) return;
-      if (has_item)
-        team_pass2((gfloat2*)(out + r.out_off), n, norm, log2F, f0, (const gfloat2*)(scratch + (size_t)b * N), t2s, wb,
-                   base, buf);
+    } else {
+      // ---- pass 2 of row i, f block blk
+      const int f0 = blk << 4;
+      {
+        const int g = threadIdx.x >> 4, k = threadIdx.x & 15;  // t2s[g][k] = W_N^{16 k (f0 + g)}
+        t2s[threadIdx.x] = tw((unsigned)((16 * k * (f0 + g)) & (N - 1)));
+      }
+      // W_N^{j f} times the row normalisation
+      const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
+      // every pass-1 item of the row has stored its part (the wait ends in a barrier)
+      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error)) return;
+      team_pass2((gfloat2*)(out + r.out_off), n, log2F, f0, U, t2s, wb, base, buf);
       // every read of this buffer has landed in registers (they fed the LDS
       // exchange), so it may be reused; the output stores need not complete
       __syncthreads();

@@ -130,11 +130,18 @@ int fcwt_hip_plan_profile(fcwt_hip_plan plan, int enable);
 
 /* Waits for the recorded events and reports, per kernel kind that ran
  * ("forward_fft", "row_writer_direct", "row_spreader",
- * "row_writer_two_pass"), the summed milliseconds and launch count since the
- * last fcwt_hip_plan_profile.  Writes at most `max` entries; *count gets the
- * number of kinds that ran. */
+ * "row_writer_two_pass", "row_team"), the summed milliseconds and launch
+ * count since the last fcwt_hip_plan_profile.  Writes at most `max` entries;
+ * *count gets the number of kinds that ran. */
 int fcwt_hip_plan_profile_read(fcwt_hip_plan plan, int max, const char** names, double* ms, long long* launches,
                                int* count);
+
+/* How the plan splits its fn rows over the row kernels: rows written by the
+ * single-work-group kernel ("row_writer_direct"), by the XCD-team kernel
+ * ("row_team") and by the two-pass fallback ("row_spreader" +
+ * "row_writer_two_pass").  Any pointer may be NULL.  Benchmarks use it to
+ * price each kernel's output bytes; no reference counterpart. */
+int fcwt_hip_plan_row_split(fcwt_hip_plan plan, int* direct_rows, int* team_rows, int* two_pass_rows);
 
 /* Bytes of device memory the plan holds (excluding caller buffers). */
 int fcwt_hip_plan_device_bytes(fcwt_hip_plan plan, size_t* bytes);

@@ -98,6 +98,27 @@ def test_headline_100k_x_3000_rows():
     sc = O.scales(O.LOGSCALES, 1000, 1, 450, 3000)
     rows = sorted(set(np.linspace(0, 2999, 40).astype(int)) | {0, 1, 628, 629, 2999})
     _check(x, sc, 2.0, rows=rows)
+    split = fcwt_amd.Plan(100_000, sc, 2.0).row_split()
+    assert split == {"direct": 671, "team": 2329, "two_pass": 0}, split
+
+
+@pytest.mark.parametrize("teams", [True, False])
+@pytest.mark.parametrize("n", [20_000, 50_000, 100_000])
+def test_wide_band_paths(n, teams, monkeypatch):
+    """N = 2^15, 2^16, 2^17 with bands up to N/2, through the XCD-team kernel
+    and (with the team path switched off) the two-pass fallback."""
+    if not teams:
+        monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
+    x = tones_noise(n, seed=n)
+    sc = O.scales(O.LOGSCALES, 1000, 0.5, 499, 48)
+    p = fcwt_amd.Plan(n, sc, 2.0)
+    split = p.row_split()
+    assert sum(split.values()) == 48
+    assert (split["team"] > 0) == teams and (split["two_pass"] > 0) == (not teams), split
+    got = run_device(x, sc, 2.0, plan=p)
+    ref = O.cwt_fp64(x, sc, 2.0, 1, True, True)
+    err = O.rel_l2_rows(got, ref)
+    assert err.max() <= TOL, f"row {int(np.argmax(err))}: rel-L2 {err.max():.3e}"
 
 
 def test_on_bin_cosine_closed_form():
