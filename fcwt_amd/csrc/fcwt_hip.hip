@@ -592,14 +592,25 @@ static int check_device(int* dev) {
 // The team path launches kTeamWgPerCu work-groups per CU cooperatively and
 // forms teams from the work-groups each XCD receives: it needs the full 256-CU device in
 // one partition.  FCWT_HIP_NO_TEAMS=1 in the environment turns it off.
-static bool team_capable(int dev) {
+static const void* team_kernel(int log2F) {
+  switch (log2F) {
+    case 7: return (const void*)k_rows_team<7>;
+    case 8: return (const void*)k_rows_team<8>;
+    case 9: return (const void*)k_rows_team<9>;
+    default: return nullptr;
+  }
+}
+
+static bool team_capable(int dev, int log2N) {
   const char* env = std::getenv("FCWT_HIP_NO_TEAMS");
   if (env && env[0] == '1') return false;
+  const void* kern = team_kernel(log2N - kTeamLog2B);
+  if (!kern) return false;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
   if (prop.multiProcessorCount != 256 || !prop.cooperativeLaunch) return false;
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_rows_team, kTeamThreads, 0) != hipSuccess) return false;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTeamThreads, 0) != hipSuccess) return false;
   return per_cu >= kTeamWgPerCu;
 }
 
@@ -700,7 +711,7 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStrea
     ta.sync = p->d_team_sync;
     ta.error = p->d_error;
     void* args[] = {&ta};
-    HIP_TRY(hipLaunchCooperativeKernel((const void*)k_rows_team, dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args,
+    HIP_TRY(hipLaunchCooperativeKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args,
                                        0, st));
   }
   for (const auto& bt : p->batches) {
@@ -805,7 +816,7 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   p->threads = threads;
   p->normalize = normalize != 0;
   p->flags = flags;
-  p->team_capable = team_capable(dev);
+  p->team_capable = team_capable(dev, log2N);
   p->scales.assign(sc, sc + fn);
 
   // host copy of the mother table, used only to judge which stale bins matter

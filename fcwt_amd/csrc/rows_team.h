@@ -56,7 +56,7 @@ struct TeamArgs {
   int seq_stride;
   int nteams;              // teams on the whole device (one per XCD)
   int team_size;           // work-groups per team
-  int log2F;
+  int log2F;               // F = N / 256 (the kernel's template parameter)
   int nbuf;                // scratch buffers per team (2..kTeamBuffers)
   float2* scratch;         // [team][nbuf][N]
   unsigned* sync;          // zeroed before launch; see below
@@ -77,6 +77,16 @@ __device__ __forceinline__ void add_agent(unsigned* p, unsigned v) {
 __device__ __forceinline__ float2 ld_agent2(const gfloat2* p) {
   unsigned long long u = __hip_atomic_load(reinterpret_cast<const gu64*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
+}
+
+// Orders this wavefront's LDS writes before its later LDS reads of other
+// lanes' values: the LDS executes one wavefront's instructions in order, so
+// only the compiler must be kept from moving memory operations across.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_sched_barrier(0);  // and keeps the register DFTs on either side apart
 }
 
 // Thread 0 waits until *p >= target (bounded); the result is broadcast.
@@ -133,25 +143,41 @@ static_assert(team_buf_need(7) <= kTeamBuf && team_buf_need(8) <= kTeamBuf && te
                   (1 << 9) * ((1 << 3) + 1) <= kTeamBuf && 4352 <= kTeamBuf,
               "team LDS buffer too small");
 
+// The team kernel's LDS, at namespace scope so that the pass functions (not
+// inlined) address it as LDS rather than through generic pointers.
+//   tm_buf: the staged band, pass-1 exchange and tile, pass-2 exchange
+//           (team_zs_offset, team_buf_need)
+__shared__ float2 tm_buf[kTeamBuf];
+__shared__ float2 tm_pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
+__shared__ float2 tm_post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
+__shared__ float2 tm_t2[kTeamB];                // pass-2 [g][k]
+__shared__ float2 tm_wb[kTeamB];                // pass-2 [k1][j]
+__shared__ SegD tm_seg[kTeamSegLds];            // the row's segments
+
 // Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A.
-//   zs  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
-//   pre [k][l']       = W_F^{P2 k l'}             (input factor)
-//   post[k1][l'][j]   = W_F^{j (l' + (F/A) k1)}  (between the register passes, P2 > 1)
+//   band (tm_buf + zs_off)  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
+//   tm_pre[k][l']           = W_F^{P2 k l'}             (input factor)
+//   tm_post[k1][l'][j]      = W_F^{j (l' + (F/A) k1)}  (between the register passes, P2 > 1)
 // Result U[f][qb0 .. qb0+Qb), stored through an LDS tile in Qb-element runs.
 // The passes are separate (not inlined) functions: inlined into the row loop
 // together they exceed the register file and spill.
-template <int LOG2A>
-__device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict__ U, const float2* zs,
-                                        const float2* pre, const float2* post, float2* buf) {
+template <int LOG2A, int LOG2F>
+__device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
+  constexpr int log2F = LOG2F;
+  constexpr int zs_off = team_zs_offset(LOG2A, LOG2F);
+  float2* const buf = tm_buf;
+  const float2* const zs = tm_buf + zs_off;
+  const float2* const pre = tm_pre;
+  const float2* const post = tm_post;
   constexpr int A = 1 << LOG2A;
   constexpr int LOG2R1 = team_log2r1(LOG2A);
   constexpr int R1 = 1 << LOG2R1;
   constexpr int LOG2P2 = LOG2A - LOG2R1;
   constexpr int P2 = 1 << LOG2P2;
   constexpr int ZS = team_zs_stride(LOG2A);
-  const int log2Ph = log2F - LOG2A;  // phases l' per qb
-  const int log2Qb = 12 - log2F;
-  const int TS = (1 << log2Qb) + 1;  // padded tile row
+  constexpr int log2Ph = log2F - LOG2A;  // phases l' per qb
+  constexpr int log2Qb = 12 - log2F;
+  constexpr int TS = (1 << log2Qb) + 1;  // padded tile row
   if constexpr (P2 == 1) {
     // 4096/A transforms of A <= 16 points: 16/A rounds of 256
 #pragma unroll 1
@@ -175,19 +201,31 @@ __device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict_
 #pragma unroll
     for (int k = 0; k < R1; ++k) v[k] = cmul(zs[ql * ZS + j + P2 * k], pre[(k << log2Ph) + lp]);
     dft_reg<R1, 1>(v);
+    // The table reads below must not be hoisted above the DFT (the compiler
+    // would hold 16 more values in registers and lose an occupancy step):
+    // their index depends on an opaque zero produced after the DFT results.
+    int after = 0;
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], post[(((k1 << log2Ph) + lp) << LOG2P2) + j]);
-    __syncthreads();  // every read of the band (same LDS) is done
+    for (int k = 0; k < R1; ++k) asm volatile("" : "+v"(v[k].x), "+v"(v[k].y));
+    asm volatile("" : "+v"(after));
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[tr * XS + j * (R1 + 1) + k1] = v[k1];
-    __syncthreads();
+    for (int k1 = 0; k1 < R1; ++k1) v[k1] = cmul(v[k1], post[(((k1 << log2Ph) + lp) << LOG2P2) + j + after]);
+    // Exchange rows: transform (ql, 0) reuses band row ql, the others sit
+    // after the Qb band rows (16 values further on, against bank conflicts).
+    // Every reader of band row ql is in this wavefront (Ph P2 = F/16 <= 64
+    // consecutive threads) and has read it above, so only the wavefront's own
+    // LDS order is needed between those reads, these writes and the reads below.
+    const int xr = lp == 0 ? ql * XS : ((1 << log2Qb) + ql * ((1 << log2Ph) - 1) + lp - 1) * XS + 16;
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[xr + j * (R1 + 1) + k1] = v[k1];
+    wave_lds_order();
     constexpr int Uu = R1 / P2;
 #pragma unroll
     for (int u = 0; u < Uu; ++u) {
       const int k1 = u * P2 + j;
       float2 w2[P2];
 #pragma unroll
-      for (int jj = 0; jj < P2; ++jj) w2[jj] = buf[tr * XS + jj * (R1 + 1) + k1];
+      for (int jj = 0; jj < P2; ++jj) w2[jj] = buf[xr + jj * (R1 + 1) + k1];
       dft_reg<P2, 1>(w2);
 #pragma unroll
       for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w2[k2];
@@ -212,11 +250,14 @@ __device__ __noinline__ void team_pass1(int log2F, int qb0, gfloat2* __restrict_
 // Pass 2 for one row: 16 consecutive f starting at f0, the factor W_N^{qb f}
 // and 256-point inverse FFTs over qb = j + 16 k, written to out (the row)
 // with non-temporal stores (written once, never re-read here).
-//   t2[g][k]  = W_N^{16 k (f0 + g)}   wb[k1][j] = W_B^{j k1}
+//   tm_t2[g][k]  = W_N^{16 k (f0 + g)}   tm_wb[k1][j] = W_B^{j k1}
 //   base      = c W_N^{j (f0 + g)} for this thread, c the row normalisation
-__device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int log2F, int f0,
-                                        const gfloat2* U, const float2* t2, const float2* wb, float2 base,
-                                        float2* buf) {
+template <int LOG2F>
+__device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int f0,
+                                        const gfloat2* U, float2 base) {
+  float2* const buf = tm_buf;
+  const float2* const t2 = tm_t2;
+  const float2* const wb = tm_wb;
   constexpr int R1 = 16, P2 = 16, XS = R1 * (P2 + 1), OS = 17;
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   float2 v[R1];
@@ -230,13 +271,15 @@ __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int lo
   for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wb[k1 * P2 + j]));
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) buf[g * XS + k1 * (P2 + 1) + j] = v[k1];
-  __syncthreads();
+  wave_lds_order();  // the 16 threads of one f are in one wavefront
   float2 w[P2];
 #pragma unroll
   for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * XS + j * (P2 + 1) + jj];
   dft_reg<P2, 1>(w);
+  // m = j + 16 k2; transpose to [m][g] so that each output run is 16
+  // consecutive f (128 bytes): 32-byte runs written straight from the
+  // registers cost a fifth of the kernel's time
   __syncthreads();
-  // m = j + 16 k2; transpose to [m][g]
 #pragma unroll
   for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = w[k2];
   __syncthreads();
@@ -244,20 +287,40 @@ __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int lo
   for (int s = 0; s < (kTeamB * 16) / kTeamThreads; ++s) {
     const int e = s * kTeamThreads + threadIdx.x;
     const int m = e >> 4, gg = e & 15;
-    const long long t = (long long)(f0 + gg) + ((long long)m << log2F);
+    const long long t = (long long)(f0 + gg) + ((long long)m << LOG2F);
     if (t < n) gstore_nt(out + t, buf[m * OS + gg]);
   }
 }
 
-__global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
-  // buf: the staged band, pass-1 exchange and tile, pass-2 exchange and
-  // transpose (team_zs_offset, team_buf_need)
-  __shared__ float2 buf[kTeamBuf];
-  __shared__ float2 pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
-  __shared__ float2 post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
-  __shared__ float2 t2s[kTeamB];             // pass-2 [g][k]
-  __shared__ float2 wb[kTeamB];              // pass-2 [k1][j]
-  __shared__ SegD sseg[kTeamSegLds];
+// Calls pass 1 for the row's A = 2^log2A (A <= F/2).
+template <int LOG2F>
+__device__ __forceinline__ void team_pass1_for(int log2A, int qb0, gfloat2* U) {
+  switch (log2A) {
+    case 1: team_pass1<1, LOG2F>(qb0, U); break;
+    case 2: team_pass1<2, LOG2F>(qb0, U); break;
+    case 3: team_pass1<3, LOG2F>(qb0, U); break;
+    case 4: team_pass1<4, LOG2F>(qb0, U); break;
+    case 5: team_pass1<5, LOG2F>(qb0, U); break;
+    case 6: team_pass1<6, LOG2F>(qb0, U); break;
+    case 7:
+      if constexpr (LOG2F > 7) team_pass1<7, LOG2F>(qb0, U);
+      break;
+    default:
+      if constexpr (LOG2F > 8) team_pass1<8, LOG2F>(qb0, U);
+      break;
+  }
+}
+
+// F = 2^LOG2F = N / 256 is a template parameter, so every stride and table
+// index in the pass functions is a compile-time constant.
+template <int LOG2F>
+__global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamArgs ta) {
+  float2* const buf = tm_buf;
+  float2* const pre = tm_pre;
+  float2* const post = tm_post;
+  float2* const t2s = tm_t2;
+  float2* const wb = tm_wb;
+  SegD* const sseg = tm_seg;
   __shared__ int s_rank, s_flag;
   // plain copies of the arguments (never take the address of the kernel argument)
   const float2* __restrict__ spec = ta.k.spec;
@@ -270,7 +333,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
   const float norm = ta.k.norm;
   const int* __restrict__ team_rows = ta.team_rows;
   const int nteams = ta.nteams, team_size = ta.team_size;
-  const int log2F = ta.log2F;
+  constexpr int log2F = LOG2F;
   unsigned* const sync = ta.sync;
   unsigned* const error = ta.error;
 
@@ -292,9 +355,9 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
   if (!s_flag) return;
   const int team = xcc;
   const int member = s_rank;
-  const int F = 1 << log2F;
-  const int log2Qb = 12 - log2F;
-  const int log2I = log2F - 4;  // items per pass per row: F/16
+  constexpr int F = 1 << log2F;
+  constexpr int log2Qb = 12 - log2F;
+  constexpr int log2I = log2F - 4;  // items per pass per row: F/16
   for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) {
     const int k1 = e >> 4, j = e & 15;
     wb[e] = tw((unsigned)(k1 * j) << log2F);
@@ -370,16 +433,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
         if (e < total) zs[ql * zstride + qf] = make_float2(xs[u].x * ws[u], xs[u].y * ws[u]);
       }
       __syncthreads();
-      switch (log2A) {
-        case 1: team_pass1<1>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 2: team_pass1<2>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 3: team_pass1<3>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 4: team_pass1<4>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 5: team_pass1<5>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 6: team_pass1<6>(log2F, qb0, U, zs, pre, post, buf); break;
-        case 7: team_pass1<7>(log2F, qb0, U, zs, pre, post, buf); break;
-        default: team_pass1<8>(log2F, qb0, U, zs, pre, post, buf); break;
-      }
+      team_pass1_for<LOG2F>(log2A, qb0, U);
       // the pass-1 stores are complete (s_waitcnt) before the arrival is counted
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __syncthreads();
@@ -395,7 +449,7 @@ __global__ __launch_bounds__(kTeamThreads) void k_rows_team(TeamArgs ta) {
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
       if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error)) return;
-      team_pass2((gfloat2*)(out + r.out_off), n, log2F, f0, U, t2s, wb, base, buf);
+      team_pass2<LOG2F>((gfloat2*)(out + r.out_off), n, f0, U, base);
       // every read of this buffer has landed in registers (they fed the LDS
       // exchange), so it may be reused; the output stores need not complete
       __syncthreads();
