@@ -51,6 +51,8 @@ SIGNATURES = {
     "fcwt_hip_plan_row_split": (ctypes.c_int, [_plan_t, _c_int_p, _c_int_p, _c_int_p]),
     "fcwt_hip_cwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_cwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fcwt_hip_ccwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
+    "fcwt_hip_ccwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 _lib = None

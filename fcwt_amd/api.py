@@ -190,11 +190,19 @@ class Plan:
         return {names[i].decode(): (ms[i], launches[i]) for i in range(min(cnt.value, 8))}
 
     def execute_host(self, x: np.ndarray, out: np.ndarray) -> None:
-        nat.check(nat.lib().fcwt_hip_cwt(self._h, x.ctypes.data, out.ctypes.data), "fcwt_hip_cwt")
+        """x: n float32 (real input) or n complex64 (complex input) samples,
+        contiguous; out: fn*n complex64, contiguous."""
+        if x.dtype == np.complex64:
+            nat.check(nat.lib().fcwt_hip_ccwt(self._h, x.ctypes.data, out.ctypes.data), "fcwt_hip_ccwt")
+        else:
+            nat.check(nat.lib().fcwt_hip_cwt(self._h, x.ctypes.data, out.ctypes.data), "fcwt_hip_cwt")
 
-    def execute_device(self, x_ptr: int, out_ptr: int, stream: int = 0) -> None:
-        nat.check(nat.lib().fcwt_hip_cwt_device(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr),
-                                                ctypes.c_void_p(stream)), "fcwt_hip_cwt_device")
+    def execute_device(self, x_ptr: int, out_ptr: int, stream: int = 0, complex_input: bool = False) -> None:
+        """Device pointers: n float32 samples (or n complex64 with
+        ``complex_input``) and fn*n complex64 outputs, enqueued on ``stream``."""
+        fn = nat.lib().fcwt_hip_ccwt_device if complex_input else nat.lib().fcwt_hip_cwt_device
+        nat.check(fn(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream)),
+                  "fcwt_hip_ccwt_device" if complex_input else "fcwt_hip_cwt_device")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -242,15 +250,22 @@ class FCWT:
             self._key = key
         return self._plan
 
-    def cwt(self, input, scales, output):
-        """FCWT::cwt(float*, int, Scales*, complex<float>*, int, int) (fcwt.cpp:540-543)."""
-        x = _input_vector(input, np.float32, "FCWT.cwt")
+    def _run(self, x, scales, output, what):
         if not isinstance(scales, Scales):
-            raise TypeError("FCWT.cwt: expected a Scales object")
-        _inplace(output, np.complex64, 2, "FCWT.cwt")
+            raise TypeError(f"{what}: expected a Scales object")
+        _inplace(output, np.complex64, 2, what)
         if output.size != x.size * scales.nscales:
-            raise ValueError(f"FCWT.cwt: output holds {output.size} values, need nscales*n = "
+            raise ValueError(f"{what}: output holds {output.size} values, need nscales*n = "
                              f"{scales.nscales}*{x.size}")
         if x.size == 0:
-            raise ValueError("FCWT.cwt: empty input")
+            raise ValueError(f"{what}: empty input")
         self.plan(x.size, scales).execute_host(x, output)
+
+    def cwt(self, input, scales, output):
+        """FCWT::cwt(float*, int, Scales*, complex<float>*, int, int) (fcwt.cpp:540-543)."""
+        self._run(_input_vector(input, np.float32, "FCWT.cwt"), scales, output, "FCWT.cwt")
+
+    def ccwt(self, input, scales, output):
+        """FCWT::cwt(complex<float>*, int, Scales*, complex<float>*, int, int)
+        (fcwt.cpp:545-548; SWIG name ``ccwt``, fcwt.i:17): complex input."""
+        self._run(_input_vector(input, np.complex64, "FCWT.ccwt"), scales, output, "FCWT.ccwt")

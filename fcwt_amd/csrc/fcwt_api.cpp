@@ -108,10 +108,29 @@ void FCWT::cwt(float* pinput, int psize, std::complex<float>* poutput, Scales* s
   check(fcwt_hip_cwt(p, pinput, reinterpret_cast<float*>(poutput)), "FCWT::cwt");
 }
 
-// fcwt.cpp:540-543
-void FCWT::cwt(float* pinput, int psize, Scales* scales, std::complex<float>* poutput, int pn1, int pn2) {
+// fcwt.cpp:536-538 (core 447-529, complex branch 486-490)
+void FCWT::cwt(std::complex<float>* pinput, int psize, std::complex<float>* poutput, Scales* scales) {
+  if (!pinput || !poutput || !scales) throw std::invalid_argument("FCWT::cwt: null argument");
+  fcwt_hip_plan p = plan_for(psize, scales);
+  check(fcwt_hip_ccwt(p, reinterpret_cast<const float*>(pinput), reinterpret_cast<float*>(poutput)), "FCWT::cwt");
+}
+
+static void check_output_size(int psize, const Scales* scales, int pn1, int pn2) {
+  if (!scales) throw std::invalid_argument("FCWT::cwt: null argument");
   if ((long long)psize * scales->nscales != (long long)pn1 * pn2)
     throw std::invalid_argument("FCWT::cwt: output holds " + std::to_string((long long)pn1 * pn2) +
                                 " values, need nscales*n = " + std::to_string((long long)psize * scales->nscales));
+}
+
+// fcwt.cpp:540-543
+void FCWT::cwt(float* pinput, int psize, Scales* scales, std::complex<float>* poutput, int pn1, int pn2) {
+  check_output_size(psize, scales, pn1, pn2);
+  cwt(pinput, psize, poutput, scales);
+}
+
+// fcwt.cpp:545-548
+void FCWT::cwt(std::complex<float>* pinput, int psize, Scales* scales, std::complex<float>* poutput, int pn1,
+               int pn2) {
+  check_output_size(psize, scales, pn1, pn2);
   cwt(pinput, psize, poutput, scales);
 }

@@ -185,18 +185,25 @@ __global__ __launch_bounds__(kBlock) void k_spreader(KArgs a) {
   }
 }
 
+// Forward transform of the zero-padded input to X[0, N/2).
+//   real input (fcwt.cpp:491-500): z[j] = x[2j] + i x[2j+1], an H = N/2 point
+//     complex FFT, then the r2c post-pass;
+//   complex input (fcwt.cpp:486-490): z[j] = x[j], an H = N point complex FFT
+//     of which the first N/2 outputs are kept.
 struct FwdArgs {
-  const float* x;
-  int n;
-  float2* spec;  // X[0, H)
-  float2* ztmp;  // H (four-step only)
-  float2* atmp;  // H (four-step only)
+  const float* x;  // the input as floats (interleaved re, im for complex input)
+  int n;           // floats in x
+  float2* spec;    // X[0, N/2)
+  float2* ztmp;    // H (four-step, real input only)
+  float2* atmp;    // H (four-step only)
   int log2H;
   int log2H1, log2H2;
+  int tw_shift;    // log2(N / H): W_H^r = W_N^{r << tw_shift}
+  int complex_in;
   Twiddles tw;
 };
 
-// packed half-length sample z[j] = x[2j] + i x[2j+1] (zero past n)
+// z[j] = x[2j] + i x[2j+1] (zero past the input)
 __device__ __forceinline__ float2 packed(const FwdArgs& a, int j) {
   long long i0 = 2LL * j;
   float re = i0 < a.n ? a.x[i0] : 0.f;
@@ -218,6 +225,10 @@ __global__ __launch_bounds__(kBlock) void k_fwd_small(FwdArgs a) {
   for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(a, j);
   __syncthreads();
   lds_fft(lds, a.log2H, 1, 0, a.tw, -1.0f);
+  if (a.complex_in) {
+    for (int k = threadIdx.x; k < H / 2; k += kBlock) a.spec[k] = lds[k];
+    return;
+  }
   for (int k = threadIdx.x; k < H; k += kBlock) {
     float2 zk = lds[k], zc = lds[(H - k) & (H - 1)];
     a.spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
@@ -242,12 +253,13 @@ __global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), k1 = e >> log2G;
     int j2 = j20 + g;
-    a.atmp[(size_t)k1 * H2 + j2] = cmul(lds[e], a.tw.get((unsigned)(2 * j2 * k1), -1.0f));
+    a.atmp[(size_t)k1 * H2 + j2] = cmul(lds[e], a.tw.get((unsigned)(j2 * k1) << a.tw_shift, -1.0f));
   }
 }
 
 // Four-step pass 2: H2-point FFTs over j2 of atmp[k1*H2 + j2] for G
-// consecutive k1, stored to ztmp[k1 + H1*k2].
+// consecutive k1, stored to ztmp[k1 + H1*k2] (complex input: straight to
+// spec, first N/2 outputs only).
 __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
   __shared__ float2 lds[kLdsCap];
   const int H1 = 1 << a.log2H1, H2 = 1 << a.log2H2;
@@ -263,7 +275,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
   lds_fft(lds, a.log2H2, G, log2G, a.tw, -1.0f);
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), k2 = e >> log2G;
-    a.ztmp[(size_t)(k10 + g) + (size_t)H1 * k2] = lds[e];
+    const size_t k = (size_t)(k10 + g) + (size_t)H1 * k2;
+    if (!a.complex_in)
+      a.ztmp[k] = lds[e];
+    else if (k < ((size_t)1 << (a.log2H - 1)))
+      a.spec[k] = lds[e];
   }
 }
 
@@ -369,7 +385,6 @@ struct fcwt_hip_plan_s {
   float* d_x = nullptr;
   float2* d_out = nullptr;
   hipStream_t stream = nullptr;
-  int log2H1 = 0, log2H2 = 0;
   size_t bytes = 0;
   // per-kernel timing (fcwt_hip_plan_profile): event pairs around launches
   struct Timed {
@@ -649,29 +664,35 @@ static void launch_direct(int log2M, unsigned blocks, const KArgs& a, hipStream_
   }
 }
 
-static int launch_all(fcwt_hip_plan p, const float* d_x, float2* d_out, hipStream_t st) {
+static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2* d_out, hipStream_t st) {
   // forward transform
   FwdArgs f;
   f.x = d_x;
-  f.n = p->n;
+  f.n = complex_in ? 2 * p->n : p->n;
   f.spec = p->d_spec;
   f.ztmp = p->d_ztmp;
   f.atmp = p->d_atmp;
-  f.log2H = p->log2N - 1;
-  f.log2H1 = p->log2H1;
-  f.log2H2 = p->log2H2;
+  f.log2H = complex_in ? p->log2N : p->log2N - 1;
+  f.log2H1 = f.log2H / 2;
+  f.log2H2 = f.log2H - f.log2H1;
+  f.tw_shift = p->log2N - f.log2H;
+  f.complex_in = complex_in ? 1 : 0;
   f.tw = twiddles_of(p);
-  if (p->N >= 2) {
+  if (complex_in && p->N == 1) {
+    // N = 1: X[0, N/2) is empty; nothing reads the spectrum
+  } else if (p->N >= 2) {
     ProfScope ps(p, 0, st);
     if (f.log2H <= 14) {
       k_fwd_small<<<1, kBlock, 0, st>>>(f);
     } else {
-      int ga = 1 << (p->log2H2 - std::min(p->log2H2, 14 - p->log2H1));
-      int gb = 1 << (p->log2H1 - std::min(p->log2H1, 14 - p->log2H2));
+      int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
+      int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
       k_fwd_a<<<ga, kBlock, 0, st>>>(f);
       k_fwd_b<<<gb, kBlock, 0, st>>>(f);
-      int H = 1 << f.log2H;
-      k_fwd_post<<<(H + 255) / 256, 256, 0, st>>>(f);
+      if (!complex_in) {
+        int H = 1 << f.log2H;
+        k_fwd_post<<<(H + 255) / 256, 256, 0, st>>>(f);
+      }
     }
   }
   KArgs a;
@@ -835,12 +856,10 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(dalloc(p, &p->d_tw1, n1));
   TRY(dalloc(p, &p->d_tw2, n2));
   TRY(dalloc(p, &p->d_spec, std::max(1, N / 2)));
-  int log2H = log2N - 1;
-  if (log2H > 14) {
-    p->log2H1 = log2H / 2;
-    p->log2H2 = log2H - p->log2H1;
-    TRY(dalloc(p, &p->d_ztmp, (size_t)1 << log2H));
-    TRY(dalloc(p, &p->d_atmp, (size_t)1 << log2H));
+  // four-step temporaries: N/2 points for real input, N for complex input
+  if (log2N > 14) {
+    TRY(dalloc(p, &p->d_ztmp, (size_t)N / 2));
+    TRY(dalloc(p, &p->d_atmp, (size_t)N));
   }
   TRY(dalloc(p, &p->d_segs, p->segs.size()));
   TRY(dalloc(p, &p->d_rows, p->rows.size()));
@@ -972,30 +991,41 @@ int fcwt_hip_plan_device_bytes(fcwt_hip_plan p, size_t* bytes) {
   return FCWT_HIP_OK;
 }
 
-int fcwt_hip_cwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, void* stream) {
-  if (!p || !d_input || !d_output) return fail(FCWT_HIP_EINVAL, "fcwt_hip_cwt_device: null argument");
+static int run_device(fcwt_hip_plan p, const float* d_input, bool complex_in, float* d_output, void* stream,
+                      const char* what) {
+  if (!p || !d_input || !d_output) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
-  int rc = launch_all(p, d_input, (float2*)d_output, (hipStream_t)stream);
+  int rc = launch_all(p, d_input, complex_in, (float2*)d_output, (hipStream_t)stream);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
 }
 
-int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
-  if (!p || !input || !output) return fail(FCWT_HIP_EINVAL, "fcwt_hip_cwt: null argument");
+int fcwt_hip_cwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, void* stream) {
+  return run_device(p, d_input, false, d_output, stream, "fcwt_hip_cwt_device");
+}
+
+int fcwt_hip_ccwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, void* stream) {
+  return run_device(p, d_input, true, d_output, stream, "fcwt_hip_ccwt_device");
+}
+
+// Host-pointer execute: input copy, kernels, output copy, error-word check.
+static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float* output, const char* what) {
+  if (!p || !input || !output) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
   int rc = FCWT_HIP_OK;
   const size_t out_elems = (size_t)p->fn * p->n;
-  if (!p->d_x) rc = dalloc(p, &p->d_x, p->n);
+  const size_t in_floats = (complex_in ? 2 : 1) * (size_t)p->n;
+  if (!p->d_x) rc = dalloc(p, &p->d_x, 2 * (size_t)p->n);  // room for either input type
   if (rc == FCWT_HIP_OK && !p->d_out) rc = dalloc(p, &p->d_out, out_elems);
   if (rc == FCWT_HIP_OK) {
-    hipError_t e = hipMemcpyAsync(p->d_x, input, sizeof(float) * p->n, hipMemcpyHostToDevice, p->stream);
+    hipError_t e = hipMemcpyAsync(p->d_x, input, sizeof(float) * in_floats, hipMemcpyHostToDevice, p->stream);
     if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("input copy failed: ") + hipGetErrorString(e));
   }
-  if (rc == FCWT_HIP_OK) rc = launch_all(p, p->d_x, p->d_out, p->stream);
+  if (rc == FCWT_HIP_OK) rc = launch_all(p, p->d_x, complex_in, p->d_out, p->stream);
   if (rc == FCWT_HIP_OK) {
     hipError_t e = hipMemcpyAsync(output, p->d_out, sizeof(float2) * out_elems, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -1004,6 +1034,14 @@ int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
   if (rc == FCWT_HIP_OK && p->nteams) rc = read_error_word(p);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
+}
+
+int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
+  return run_host(p, input, false, output, "fcwt_hip_cwt");
+}
+
+int fcwt_hip_ccwt(fcwt_hip_plan p, const float* input, float* output) {
+  return run_host(p, input, true, output, "fcwt_hip_ccwt");
 }
 
 }  // extern "C"

@@ -100,9 +100,11 @@ class FCWT {
   void create_FFT_optimization_plan(int pmaxsize, int poptimizationflags);
   void create_FFT_optimization_plan(int pmaxsize, std::string poptimizationflags);
 
-  // fcwt.cpp:532-548: real input, output fn x n complex64 row-major
+  // fcwt.cpp:532-548: real or complex input, output fn x n complex64 row-major
   void cwt(float* pinput, int psize, std::complex<float>* poutput, Scales* scales);
+  void cwt(std::complex<float>* pinput, int psize, std::complex<float>* poutput, Scales* scales);
   void cwt(float* pinput, int psize, Scales* scales, std::complex<float>* poutput, int pn1, int pn2);
+  void cwt(std::complex<float>* pinput, int psize, Scales* scales, std::complex<float>* poutput, int pn1, int pn2);
 
   Wavelet* wavelet;
 

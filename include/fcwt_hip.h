@@ -157,6 +157,17 @@ int fcwt_hip_cwt(fcwt_hip_plan plan, const float* input, float* output);
  * the default stream).  Returns once enqueued.  d_output holds fn*n complex64. */
 int fcwt_hip_cwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
 
+/* FCWT::cwt(complex<float>* in, int n, complex<float>* out, Scales*),
+ * src/fcwt/fcwt.cpp:536-538 (core 447-529; complex branch 486-490: the
+ * zero-padded complex signal goes through a forward c2c FFT instead of the
+ * r2c one), host pointers: input holds n interleaved complex64 samples
+ * (2n floats), output as fcwt_hip_cwt.  Only the positive-frequency half
+ * of the spectrum enters any row (the Morlet window is one-sided). */
+int fcwt_hip_ccwt(fcwt_hip_plan plan, const float* input, float* output);
+
+/* The same with device pointers, enqueued on `stream`. */
+int fcwt_hip_ccwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

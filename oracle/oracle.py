@@ -144,10 +144,24 @@ def time_wavelet(fb: float, scale: float, pn: int) -> np.ndarray:
 
 
 # ------------------------------------------------------------ the transforms
+def _signal(x):
+    """float32 (real input) or complex64 (complex input, fcwt.cpp:486-490) copy."""
+    return np.asarray(x, np.complex64 if np.iscomplexobj(x) else np.float32)
+
+
 def forward_fp64(x: np.ndarray, N: int) -> np.ndarray:
-    """X[k] = sum_{t<n} x[t] e^{-2 pi i k t / N}, k < N/2 (fcwt.cpp:486-500), float64."""
+    """X[k] = sum_{t<n} x[t] e^{-2 pi i k t / N}, k < N/2, float64: the r2c
+    transform of the zero-padded real signal (fcwt.cpp:491-500) or the c2c
+    transform of the zero-padded complex signal (fcwt.cpp:486-490).  Only
+    k < N/2 is ever read (E_i <= N/2), so the reference's Hermitian fill of
+    the upper half (fcwt.cpp:507-510) does not matter here."""
+    x = _signal(x)
+    if np.iscomplexobj(x):
+        xp = np.zeros(N, np.complex128)
+        xp[: x.size] = x.astype(np.complex128)
+        return np.fft.fft(xp)[: N // 2]
     xp = np.zeros(N, np.float64)
-    xp[: x.size] = np.asarray(x, np.float32).astype(np.float64)
+    xp[: x.size] = x.astype(np.float64)
     return np.fft.rfft(xp)[: N // 2]
 
 
@@ -169,7 +183,8 @@ def o1_rows(X: np.ndarray, sc: np.ndarray, mom: np.ndarray, N: int, threads: int
 
 def cwt_fp64(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bool = True,
              normalize: bool = True, rows=None) -> np.ndarray:
-    """Float64 restatement of FCWT::cwt for real input (src/fcwt/fcwt.cpp:447-529).
+    """Float64 restatement of FCWT::cwt (src/fcwt/fcwt.cpp:447-529), for real
+    (float32) or complex (complex64, fcwt.cpp:536-538) input.
 
     Row i is c * IDFT_N(O1_i)[0:n] with c = 1/N when normalising (fcwt.cpp:431-445),
     where O1_i holds X[q] * mother[idx_i(q)] for q < C_i and, for q >= C_i, the
@@ -177,7 +192,7 @@ def cwt_fp64(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bo
     buffer across scales, fcwt.cpp:449, 466-467, 514-518; bins >= N/2 stay 0).
     Returns complex128 rows (all rows, or those listed in ``rows``).
     """
-    x = np.asarray(x, np.float32)
+    x = _signal(x)
     n = x.size
     N = 1 << find2power(n)
     X = forward_fp64(x, N)
@@ -201,12 +216,12 @@ def cwt_fp64(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bo
 def cwt_fp32_port(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bool = True,
                   normalize: bool = True, rows=None, out=None) -> np.ndarray:
     """The same path in float32 with numpy's float32 FFTs (CPU baseline port)."""
-    x = np.asarray(x, np.float32)
+    x = _signal(x)
     n = x.size
     N = 1 << find2power(n)
-    xp = np.zeros(N, np.float32)
+    xp = np.zeros(N, x.dtype)
     xp[:n] = x
-    X = np.fft.rfft(xp)[: N // 2].astype(np.complex64)
+    X = (np.fft.fft(xp) if np.iscomplexobj(x) else np.fft.rfft(xp))[: N // 2].astype(np.complex64)
     mom = mother(fb, N)
     sc = np.ascontiguousarray(sc, np.float32)
     nrow = sc.size if rows is None else int(rows)
@@ -228,8 +243,9 @@ def brute_force_cwt(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, 
 
     X[k] is summed explicitly; each row is summed explicitly over the bins the
     reference's O1 holds for it, including the stale bins (see ``cwt_fp64``).
+    Real or complex input.
     """
-    x = np.asarray(x, np.float32).astype(np.float64)
+    x = _signal(x).astype(np.complex128)
     n = x.size
     N = 1 << find2power(n)
     t = np.arange(n)

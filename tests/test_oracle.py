@@ -189,6 +189,43 @@ def test_forward_identities():
     np.testing.assert_allclose(O.forward_fp64(x, N), np.exp(-2j * np.pi * k * 37 / N), atol=1e-12)
 
 
+@pytest.mark.parametrize("n", [1, 3, 64, 100, 255])
+def test_forward_fp64_complex_brute_force(n):
+    """Complex input: the c2c transform of the zero-padded signal (fcwt.cpp:486-490)."""
+    rng = np.random.default_rng(100 + n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    N = 1 << O.find2power(n)
+    X = O.forward_fp64(x, N)
+    np.testing.assert_allclose(X, _dft(x, N, -1)[: N // 2], rtol=0, atol=1e-10 * max(1, np.abs(x).sum()))
+
+
+def test_cwt_fp64_complex_closed_forms():
+    """n = N.  x = e^{+2 pi i k0 t/N}: X[k0] = N, so row i is c N mother[idx_i(k0)]
+    e^{2 pi i k0 t/N} when k0 < C_i.  x = e^{-2 pi i k0 t/N} sits at bin N - k0
+    >= N/2, which no row reads (Morlet is one-sided): every row is zero.
+    A real signal given as complex gives the real-input rows."""
+    N, k0 = 2048, 200
+    t = np.arange(N)
+    sc = O.scales(O.LOGSCALES, 1000, 5, 450, 30)
+    mom = O.mother(2.0, N)
+    E, step, C = O.rows_meta(N, sc, 1, True)
+    pos = np.exp(2j * np.pi * k0 * t / N).astype(np.complex64)
+    got = O.cwt_fp64(pos, sc, 2.0)
+    checked = 0
+    for i in range(len(sc)):
+        if k0 < C[i]:
+            idx = int(min(np.float32(N - 1), step[i] * np.float32(k0)))
+            expect = float(mom[idx]) * pos.astype(np.complex128)  # c N = 1
+            assert np.abs(got[i] - expect).max() <= 1e-6 * float(mom.max()), i
+            checked += 1
+    assert checked > 3
+    neg = np.exp(-2j * np.pi * k0 * t / N).astype(np.complex64)
+    assert np.abs(O.cwt_fp64(neg, sc, 2.0)).max() <= 1e-6 * float(mom.max())
+    xr = np.random.default_rng(7).standard_normal(N - 5).astype(np.float32)
+    np.testing.assert_allclose(O.cwt_fp64(xr.astype(np.complex64), sc, 2.0), O.cwt_fp64(xr, sc, 2.0),
+                               rtol=0, atol=1e-12 * np.abs(xr).sum())
+
+
 def _tiny_problem(seed, n, fn, fb, lowcov=False):
     rng = np.random.default_rng(seed)
     x = rng.standard_normal(n).astype(np.float32)
@@ -214,6 +251,15 @@ def test_cwt_fp64_brute_force(seed, n, fn, fb, T, avx, norm, lowcov):
     # a subset of rows gives the same rows (stale bins are tracked across all rows)
     sub = O.cwt_fp64(x, sc, fb, T, avx, norm, rows=[fn - 1, 2])
     np.testing.assert_allclose(sub, got[[2, fn - 1]], rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("seed,n,fn,fb,T", [(10, 64, 10, 2.0, 1), (11, 150, 8, 1.0, 8), (12, 257, 6, 6.0, 6)])
+def test_cwt_fp64_complex_brute_force(seed, n, fn, fb, T):
+    x, sc = _tiny_problem(seed, n, fn, fb)
+    x = (x + 1j * np.random.default_rng(seed).standard_normal(n)).astype(np.complex64)
+    got = O.cwt_fp64(x, sc, fb, T)
+    ref = O.brute_force_cwt(x, sc, fb, T)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-11 * np.abs(ref).max())
 
 
 def test_cwt_on_bin_cosine_closed_form():
