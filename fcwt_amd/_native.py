@@ -53,6 +53,9 @@ SIGNATURES = {
     "fcwt_hip_cwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_ccwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_ccwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fcwt_hip_adjoint_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                               ctypes.c_void_p]),
+    "fcwt_hip_adjoint": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
 }
 
 _lib = None

@@ -204,6 +204,24 @@ class Plan:
         nat.check(fn(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream)),
                   "fcwt_hip_ccwt_device" if complex_input else "fcwt_hip_cwt_device")
 
+    def adjoint_device(self, g_ptr: int, xbar_ptr: int, stream: int = 0, complex_input: bool = False) -> None:
+        """Input gradient from the output gradient (device pointers): g holds
+        fn*n complex64 (dL/dRe W + i dL/dIm W), xbar receives n float32 (or n
+        complex64 with ``complex_input``).  See fcwt_hip_adjoint_device."""
+        nat.check(nat.lib().fcwt_hip_adjoint_device(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(xbar_ptr),
+                                                    int(bool(complex_input)), ctypes.c_void_p(stream)),
+                  "fcwt_hip_adjoint_device")
+
+    def adjoint_host(self, g: np.ndarray, complex_input: bool = False) -> np.ndarray:
+        """Host-array form of ``adjoint_device``: g is (fn, n) complex64."""
+        g = np.ascontiguousarray(g, np.complex64)
+        if g.size != self.fn * self.n:
+            raise ValueError(f"adjoint: gradient holds {g.size} values, need fn*n = {self.fn}*{self.n}")
+        xbar = np.empty(self.n, np.complex64 if complex_input else np.float32)
+        nat.check(nat.lib().fcwt_hip_adjoint(self._h, g.ctypes.data, xbar.ctypes.data, int(bool(complex_input))),
+                  "fcwt_hip_adjoint")
+        return xbar
+
     def close(self):
         if getattr(self, "_h", None):
             nat.lib().fcwt_hip_plan_destroy(self._h)

@@ -1,0 +1,81 @@
+"""Differentiable CWT on device tensors: a ``torch.autograd.Function`` over
+the plan's forward (fcwt_hip_cwt_device / fcwt_hip_ccwt_device) and its
+hand-written adjoint (fcwt_hip_adjoint_device).
+
+The reference has no derivative; the forward it differentiates is
+FCWT::cwt (src/fcwt/fcwt.cpp:447-548).  For a real loss L of the output W,
+PyTorch hands ``backward`` the gradient G = dL/dRe W + i dL/dIm W, and the
+input gradient is
+
+    xbar[t] = c Re sum_{q<N/2} A[q] e^{2 pi i q t / N},
+    A[q]    = sum_i w_i(q) FFT_N(G_i)[q]
+
+(the complex value itself for complex input), computed on the GPU; the signal
+spectrum is not stored between forward and backward.
+
+Usage::
+
+    op = CWT(n, scales, fb=2.0)        # one plan, reused
+    W = op(x)                          # x: (n,) or (B, n) float32/complex64 cuda tensor
+    loss = (w * W.abs() ** 2).sum(); loss.backward()
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .api import Plan
+
+
+def _stream(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+class _CWTFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x: torch.Tensor, plan: Plan):
+        if not x.is_cuda:
+            raise ValueError("CWT: the input must be a CUDA (HIP) tensor")
+        cplx = x.is_complex()
+        xin = x.detach().to(torch.complex64 if cplx else torch.float32).contiguous()
+        batch = xin.shape[:-1]
+        rows = xin.reshape(-1, plan.n)
+        out = torch.empty((rows.shape[0], plan.fn, plan.n), dtype=torch.complex64, device=x.device)
+        st = _stream(x)
+        for b in range(rows.shape[0]):
+            plan.execute_device(rows[b].data_ptr(), out[b].data_ptr(), st, complex_input=cplx)
+        ctx.plan = plan
+        ctx.cplx = cplx
+        ctx.in_dtype = x.dtype
+        return out.reshape(*batch, plan.fn, plan.n)
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        plan, cplx = ctx.plan, ctx.cplx
+        g = grad_out.to(torch.complex64).contiguous().reshape(-1, plan.fn, plan.n)
+        xbar = torch.empty((g.shape[0], plan.n), dtype=torch.complex64 if cplx else torch.float32,
+                           device=grad_out.device)
+        st = _stream(grad_out)
+        for b in range(g.shape[0]):
+            plan.adjoint_device(g[b].data_ptr(), xbar[b].data_ptr(), st, complex_input=cplx)
+        shape = grad_out.shape[:-2] + (plan.n,)
+        return xbar.reshape(shape).to(ctx.in_dtype), None
+
+
+class CWT(torch.nn.Module):
+    """The Morlet CWT of signals of length ``n`` at ``scales`` as a
+    differentiable module; output (..., fn, n) complex64."""
+
+    def __init__(self, n: int, scales, fb: float = 2.0, normalize: bool = True, threads: int = 1):
+        super().__init__()
+        self.plan = Plan(int(n), np.ascontiguousarray(scales, np.float32), float(fb), threads, normalize)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        if x.shape[-1] != self.plan.n:
+            raise ValueError(f"CWT: signal length {x.shape[-1]} != planned n = {self.plan.n}")
+        return _CWTFunction.apply(x, self.plan)
+
+
+def cwt(x: torch.Tensor, plan: Plan) -> torch.Tensor:
+    """Functional form: the plan's CWT of ``x`` with autograd support."""
+    return _CWTFunction.apply(x, plan)

@@ -40,7 +40,7 @@ namespace fcwt_hip {
 constexpr int kMaxLog2B = 10;  // writer FFT size cap: G * B <= kLdsCap with G >= 16
 constexpr double kStaleFloor = 1e-12;
 constexpr size_t kScratchBytes = size_t(256) << 20;
-constexpr int kKinds = 5;  // kernel kinds timed by fcwt_hip_plan_profile
+constexpr int kKinds = 6;  // kernel kinds timed by fcwt_hip_plan_profile
 
 struct SegD {
   int lo, hi;
@@ -189,13 +189,18 @@ __global__ __launch_bounds__(kBlock) void k_spreader(KArgs a) {
 //   real input (fcwt.cpp:491-500): z[j] = x[2j] + i x[2j+1], an H = N/2 point
 //     complex FFT, then the r2c post-pass;
 //   complex input (fcwt.cpp:486-490): z[j] = x[j], an H = N point complex FFT
-//     of which the first N/2 outputs are kept.
+//     of which the first `keep` outputs are kept (N/2 for the cwt).
+// Several signals at once: grid dimension y indexes them, each at its own
+// stride in x, spec and atmp.
 struct FwdArgs {
   const float* x;  // the input as floats (interleaved re, im for complex input)
-  int n;           // floats in x
-  float2* spec;    // X[0, N/2)
+  int n;           // floats in x per signal
+  float2* spec;    // X[0, keep) (real input: X[0, N/2))
   float2* ztmp;    // H (four-step, real input only)
-  float2* atmp;    // H (four-step only)
+  float2* atmp;    // H per signal (four-step only)
+  long long x_stride;     // floats between signals
+  long long spec_stride;  // complex values between signals' outputs
+  int keep;               // complex input: outputs kept
   int log2H;
   int log2H1, log2H2;
   int tw_shift;    // log2(N / H): W_H^r = W_N^{r << tw_shift}
@@ -203,11 +208,11 @@ struct FwdArgs {
   Twiddles tw;
 };
 
-// z[j] = x[2j] + i x[2j+1] (zero past the input)
-__device__ __forceinline__ float2 packed(const FwdArgs& a, int j) {
+// z[j] = x[2j] + i x[2j+1] (zero past the n floats of x)
+__device__ __forceinline__ float2 packed(const float* x, int n, int j) {
   long long i0 = 2LL * j;
-  float re = i0 < a.n ? a.x[i0] : 0.f;
-  float im = i0 + 1 < a.n ? a.x[i0 + 1] : 0.f;
+  float re = i0 < n ? x[i0] : 0.f;
+  float im = i0 + 1 < n ? x[i0 + 1] : 0.f;
   return make_float2(re, im);
 }
 
@@ -222,16 +227,18 @@ __device__ __forceinline__ float2 real_post(float2 zk, float2 zc, float2 w) {
 __global__ __launch_bounds__(kBlock) void k_fwd_small(FwdArgs a) {
   __shared__ float2 lds[kLdsCap];
   const int H = 1 << a.log2H;
-  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(a, j);
+  const float* x = a.x + blockIdx.y * a.x_stride;
+  float2* spec = a.spec + blockIdx.y * a.spec_stride;
+  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(x, a.n, j);
   __syncthreads();
   lds_fft(lds, a.log2H, 1, 0, a.tw, -1.0f);
   if (a.complex_in) {
-    for (int k = threadIdx.x; k < H / 2; k += kBlock) a.spec[k] = lds[k];
+    for (int k = threadIdx.x; k < a.keep; k += kBlock) spec[k] = lds[k];
     return;
   }
   for (int k = threadIdx.x; k < H; k += kBlock) {
     float2 zk = lds[k], zc = lds[(H - k) & (H - 1)];
-    a.spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
+    spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
   }
 }
 
@@ -244,16 +251,18 @@ __global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
   const int G = 1 << log2G;
   const int j20 = blockIdx.x << log2G;
   const int tot = H1 << log2G;
+  const float* x = a.x + blockIdx.y * a.x_stride;
+  float2* atmp = a.atmp + ((size_t)blockIdx.y << a.log2H);
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), j1 = e >> log2G;
-    lds[e] = packed(a, j20 + g + H2 * j1);
+    lds[e] = packed(x, a.n, j20 + g + H2 * j1);
   }
   __syncthreads();
   lds_fft(lds, a.log2H1, G, log2G, a.tw, -1.0f);
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), k1 = e >> log2G;
     int j2 = j20 + g;
-    a.atmp[(size_t)k1 * H2 + j2] = cmul(lds[e], a.tw.get((unsigned)(j2 * k1) << a.tw_shift, -1.0f));
+    atmp[(size_t)k1 * H2 + j2] = cmul(lds[e], a.tw.get((unsigned)(j2 * k1) << a.tw_shift, -1.0f));
   }
 }
 
@@ -267,9 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
   const int G = 1 << log2G;
   const int k10 = blockIdx.x << log2G;
   const int tot = H2 << log2G;
+  const float2* atmp = a.atmp + ((size_t)blockIdx.y << a.log2H);
+  float2* spec = a.spec + blockIdx.y * a.spec_stride;
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int j2 = e & (H2 - 1), g = e >> a.log2H2;
-    lds[(j2 << log2G) + g] = a.atmp[(size_t)(k10 + g) * H2 + j2];
+    lds[(j2 << log2G) + g] = atmp[(size_t)(k10 + g) * H2 + j2];
   }
   __syncthreads();
   lds_fft(lds, a.log2H2, G, log2G, a.tw, -1.0f);
@@ -278,8 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
     const size_t k = (size_t)(k10 + g) + (size_t)H1 * k2;
     if (!a.complex_in)
       a.ztmp[k] = lds[e];
-    else if (k < ((size_t)1 << (a.log2H - 1)))
-      a.spec[k] = lds[e];
+    else if (k < (size_t)a.keep)
+      spec[k] = lds[e];
   }
 }
 
@@ -287,6 +298,57 @@ __global__ void k_fwd_post(FwdArgs a) {
   const int H = 1 << a.log2H;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < H) a.spec[k] = real_post(a.ztmp[k], a.ztmp[(H - k) & (H - 1)], a.tw.get((unsigned)k, -1.0f));
+}
+
+// ---- adjoint (input gradient) kernels
+// Window the forward used for bin q of a row: mother[(int)fminf(N-1, step q)]
+// in the row's segment holding q, 0 outside (the same segments as the forward).
+__device__ __forceinline__ float row_window(const SegD* segs, const RowD& r, int q, float maxidx,
+                                            const float* mother) {
+#pragma clang fp contract(off)
+  for (int s = 0; s < r.seg_count; ++s) {
+    const SegD sg = segs[r.seg_begin + s];
+    if (q >= sg.lo && q < sg.hi) {
+      float v = sg.step * (float)q;
+      return mother[(int)(v < maxidx ? v : maxidx)];
+    }
+  }
+  return 0.f;
+}
+
+// A[q] (+)= sum over rows [row0, row0 + count) of w_i(q) Ghat_i[q], q < N/2,
+// with Ghat_i = P[i - row0][0, N/2); on the last chunk A is stored conjugated,
+// ready for the final forward transform.
+__global__ void k_adj_reduce(const float2* P, int row0, int count, const RowD* rows, const SegD* segs,
+                             const float* mother, int N, float2* A, int first, int last) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int half = N / 2;
+  if (q >= half) return;
+  const float maxidx = (float)N - 1.0f;
+  float2 acc = first ? make_float2(0.f, 0.f) : A[q];
+  for (int i = 0; i < count; ++i) {
+    const float w = row_window(segs, rows[row0 + i], q, maxidx, mother);
+    if (w != 0.f) {
+      const float2 g = P[(size_t)i * half + q];
+      acc.x += w * g.x;
+      acc.y += w * g.y;
+    }
+  }
+  A[q] = last ? make_float2(acc.x, -acc.y) : acc;
+}
+
+// xbar[t] = c Re y[t] (real input) or c conj(y[t]) (complex input), t < n,
+// where y = FFT_N(conj A): c Re / c of the inverse transform of A.
+__global__ void k_adj_out(const float2* y, int n, float c, int complex_in, float* xbar) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float2 v = y[t];
+  if (complex_in) {
+    xbar[2 * t] = c * v.x;
+    xbar[2 * t + 1] = -c * v.y;
+  } else {
+    xbar[t] = c * v.x;
+  }
 }
 
 // Mother table (fcwt.cpp:51-67): the argument in float32 exactly as the
@@ -384,6 +446,14 @@ struct fcwt_hip_plan_s {
   float2* d_scratch = nullptr;
   float* d_x = nullptr;
   float2* d_out = nullptr;
+  // adjoint workspace (allocated by the first adjoint call)
+  int adj_chunk = 0;
+  float2* d_adj_P = nullptr;     // adj_chunk x N/2 row spectra
+  float2* d_adj_atmp = nullptr;  // adj_chunk x N four-step temporaries (N > 2^14)
+  float2* d_adj_A = nullptr;     // N/2
+  float2* d_adj_y = nullptr;     // N
+  float* d_grad = nullptr;       // host-pointer adjoint: fn x n complex64 staging
+  float* d_xbar = nullptr;
   hipStream_t stream = nullptr;
   size_t bytes = 0;
   // per-kernel timing (fcwt_hip_plan_profile): event pairs around launches
@@ -399,7 +469,8 @@ struct fcwt_hip_plan_s {
 };
 
 // kernel kinds reported by fcwt_hip_plan_profile_read
-static const char* const kKindNames[kKinds] = {"forward_fft", "row_writer_direct", "row_spreader", "row_writer_two_pass", "row_team"};
+static const char* const kKindNames[kKinds] = {"forward_fft", "row_writer_direct", "row_spreader",
+                                               "row_writer_two_pass", "row_team", "adjoint"};
 
 static hipEvent_t take_event(fcwt_hip_plan p) {
   if (!p->event_pool.empty()) {
@@ -453,7 +524,8 @@ static void free_plan(fcwt_hip_plan p) {
   for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
-                  p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error};
+                  p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
+                  p->d_adj_P, p->d_adj_atmp, p->d_adj_A, p->d_adj_y, p->d_grad, p->d_xbar};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -664,6 +736,84 @@ static void launch_direct(int log2M, unsigned blocks, const KArgs& a, hipStream_
   }
 }
 
+// Forward transforms of `batch` complex signals of f.n floats each (strides
+// in f), keeping f.keep outputs: one work-group per signal up to 2^14 points,
+// the two four-step kernels beyond.
+static void launch_fwd_complex(FwdArgs f, int batch, hipStream_t st) {
+  f.complex_in = 1;
+  f.log2H1 = f.log2H / 2;
+  f.log2H2 = f.log2H - f.log2H1;
+  if (f.log2H <= 14) {
+    k_fwd_small<<<dim3(1, batch), kBlock, 0, st>>>(f);
+  } else {
+    int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
+    int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
+    k_fwd_a<<<dim3(ga, batch), kBlock, 0, st>>>(f);
+    k_fwd_b<<<dim3(gb, batch), kBlock, 0, st>>>(f);
+  }
+}
+
+constexpr size_t kAdjScratchBytes = 256u << 20;  // adjoint row-spectrum workspace
+
+static int adjoint_workspace(fcwt_hip_plan p) {
+  if (p->d_adj_y) return FCWT_HIP_OK;
+  const size_t N = (size_t)p->N;
+  const size_t per_row = sizeof(float2) * (N / 2 + (p->log2N > 14 ? N : 0));
+  p->adj_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->fn, kAdjScratchBytes / per_row));
+  int rc = dalloc(p, &p->d_adj_P, (size_t)p->adj_chunk * (N / 2));
+  if (rc == FCWT_HIP_OK && p->log2N > 14) rc = dalloc(p, &p->d_adj_atmp, (size_t)p->adj_chunk * N);
+  if (rc == FCWT_HIP_OK) rc = dalloc(p, &p->d_adj_A, N / 2);
+  if (rc == FCWT_HIP_OK) rc = dalloc(p, &p->d_adj_y, N);
+  return rc;
+}
+
+// Input gradient from the output gradient d_grad (fn rows of n complex64):
+// per chunk of rows, Ghat_i = FFT_N(G_i)[0, N/2) and A += w_i Ghat_i; then
+// xbar = c Re IFFT_N(A)[0, n) (or the complex value for complex input).
+static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, bool complex_in, float* d_xbar, hipStream_t st) {
+  const int n = p->n, N = p->N;
+  const float c = p->normalize ? 1.0f / (float)N : 1.0f;
+  if (N < 2) {  // no spectrum bin: the gradient is zero
+    k_fill_u32<<<(unsigned)((2 * n + 255) / 256), 256, 0, st>>>((unsigned*)d_xbar, (size_t)(complex_in ? 2 : 1) * n,
+                                                               0u);
+    HIP_TRY(hipGetLastError());
+    return FCWT_HIP_OK;
+  }
+  int rc = adjoint_workspace(p);
+  if (rc != FCWT_HIP_OK) return rc;
+  FwdArgs f;
+  f.n = 2 * n;
+  f.x_stride = 2LL * n;
+  f.spec_stride = N / 2;
+  f.keep = N / 2;
+  f.ztmp = nullptr;
+  f.atmp = p->d_adj_atmp;
+  f.log2H = p->log2N;
+  f.tw_shift = 0;
+  f.tw = twiddles_of(p);
+  ProfScope ps(p, 5, st);
+  for (int row0 = 0; row0 < p->fn; row0 += p->adj_chunk) {
+    const int count = std::min(p->adj_chunk, p->fn - row0);
+    f.x = d_grad + (size_t)row0 * 2 * n;
+    f.spec = p->d_adj_P;
+    launch_fwd_complex(f, count, st);
+    k_adj_reduce<<<(N / 2 + 255) / 256, 256, 0, st>>>(p->d_adj_P, row0, count, p->d_rows, p->d_segs, p->d_mother, N,
+                                                      p->d_adj_A, row0 == 0, row0 + count >= p->fn);
+  }
+  // y = FFT_N(conj A), the first n outputs
+  f.x = (const float*)p->d_adj_A;
+  f.n = N;  // N/2 complex values
+  f.x_stride = 0;
+  f.spec = p->d_adj_y;
+  f.spec_stride = 0;
+  f.keep = n;
+  f.atmp = p->d_adj_atmp;
+  launch_fwd_complex(f, 1, st);
+  k_adj_out<<<(n + 255) / 256, 256, 0, st>>>(p->d_adj_y, n, c, complex_in ? 1 : 0, d_xbar);
+  HIP_TRY(hipGetLastError());
+  return FCWT_HIP_OK;
+}
+
 static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2* d_out, hipStream_t st) {
   // forward transform
   FwdArgs f;
@@ -677,6 +827,9 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2
   f.log2H2 = f.log2H - f.log2H1;
   f.tw_shift = p->log2N - f.log2H;
   f.complex_in = complex_in ? 1 : 0;
+  f.x_stride = 0;
+  f.spec_stride = 0;
+  f.keep = p->N / 2;
   f.tw = twiddles_of(p);
   if (complex_in && p->N == 1) {
     // N = 1: X[0, N/2) is empty; nothing reads the spectrum
@@ -1038,6 +1191,40 @@ static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float*
 
 int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
   return run_host(p, input, false, output, "fcwt_hip_cwt");
+}
+
+int fcwt_hip_adjoint_device(fcwt_hip_plan p, const float* d_grad, float* d_xbar, int complex_input, void* stream) {
+  if (!p || !d_grad || !d_xbar) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint_device: null argument");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  int rc = launch_adjoint(p, d_grad, complex_input != 0, d_xbar, (hipStream_t)stream);
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
+}
+
+int fcwt_hip_adjoint(fcwt_hip_plan p, const float* grad, float* xbar, int complex_input) {
+  if (!p || !grad || !xbar) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint: null argument");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  int rc = FCWT_HIP_OK;
+  const size_t g_floats = 2 * (size_t)p->fn * p->n;
+  const size_t x_floats = (complex_input ? 2 : 1) * (size_t)p->n;
+  if (!p->d_grad) rc = dalloc(p, &p->d_grad, g_floats);
+  if (rc == FCWT_HIP_OK && !p->d_xbar) rc = dalloc(p, &p->d_xbar, 2 * (size_t)p->n);
+  if (rc == FCWT_HIP_OK) {
+    hipError_t e = hipMemcpyAsync(p->d_grad, grad, sizeof(float) * g_floats, hipMemcpyHostToDevice, p->stream);
+    if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("gradient copy failed: ") + hipGetErrorString(e));
+  }
+  if (rc == FCWT_HIP_OK) rc = launch_adjoint(p, p->d_grad, complex_input != 0, p->d_xbar, p->stream);
+  if (rc == FCWT_HIP_OK) {
+    hipError_t e = hipMemcpyAsync(xbar, p->d_xbar, sizeof(float) * x_floats, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("adjoint failed: ") + hipGetErrorString(e));
+  }
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
 }
 
 int fcwt_hip_ccwt(fcwt_hip_plan p, const float* input, float* output) {

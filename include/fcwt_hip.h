@@ -130,7 +130,7 @@ int fcwt_hip_plan_profile(fcwt_hip_plan plan, int enable);
 
 /* Waits for the recorded events and reports, per kernel kind that ran
  * ("forward_fft", "row_writer_direct", "row_spreader",
- * "row_writer_two_pass", "row_team"), the summed milliseconds and launch
+ * "row_writer_two_pass", "row_team", "adjoint"), the summed milliseconds and launch
  * count since the last fcwt_hip_plan_profile.  Writes at most `max` entries;
  * *count gets the number of kinds that ran. */
 int fcwt_hip_plan_profile_read(fcwt_hip_plan plan, int max, const char** names, double* ms, long long* launches,
@@ -167,6 +167,22 @@ int fcwt_hip_ccwt(fcwt_hip_plan plan, const float* input, float* output);
 
 /* The same with device pointers, enqueued on `stream`. */
 int fcwt_hip_ccwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
+
+/* Adjoint of the plan's transform: the input gradient from an output
+ * gradient (no reference counterpart -- the reference has no derivative; the
+ * torch.autograd.Function of fcwt_amd uses this).  d_grad holds fn*n complex64,
+ * G_i[t] = dL/dRe W_i[t] + i dL/dIm W_i[t]; d_xbar receives n float32
+ * (complex_input = 0: dL/dx for real input) or n interleaved complex64
+ * (complex_input != 0: dL/dRe x + i dL/dIm x):
+ *   xbar[t] = c sum_{q<N/2} A[q] e^{2 pi i q t/N}   (real part for real input)
+ *   A[q]    = sum_i w_i(q) sum_{t'<n} G_i[t'] e^{-2 pi i q t'/N}
+ * with w_i the window the forward applied to row i (stale bins included) and
+ * c = 1/N when normalising.  Enqueued on `stream`; the first call allocates
+ * the plan's adjoint workspace (about 256 MiB at most). */
+int fcwt_hip_adjoint_device(fcwt_hip_plan plan, const float* d_grad, float* d_xbar, int complex_input, void* stream);
+
+/* The same with host pointers; returns when xbar is complete. */
+int fcwt_hip_adjoint(fcwt_hip_plan plan, const float* grad, float* xbar, int complex_input);
 
 #ifdef __cplusplus
 }

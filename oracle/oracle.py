@@ -237,6 +237,42 @@ def cwt_fp32_port(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, av
     return out
 
 
+def effective_windows(sc: np.ndarray, fb: float, N: int, threads: int = 1, avx: bool = True):
+    """Yields (i, w_i): the window each bin of row i is effectively multiplied
+    by, w_i[q] = mother[idx_j(q)] of the last scale j <= i that covered q
+    (the reference's reused O1, fcwt.cpp:449, 514-518), length N/2, float64.
+    The array is reused between iterations."""
+    mom = mother(fb, N)
+    W = np.zeros(N // 2, np.float64)
+    for i, s in enumerate(np.asarray(sc, np.float32)):
+        w, C = row_window(mom, float(s), N, threads, avx)
+        W[:C] = w[:C]
+        yield i, W
+
+
+def adjoint_fp64(G: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bool = True,
+                 normalize: bool = True, complex_input: bool = False) -> np.ndarray:
+    """Adjoint of ``cwt_fp64`` (no reference counterpart; the forward is
+    fcwt.cpp:447-548).  G is (fn, n): G_i[t] = dL/dRe W_i[t] + i dL/dIm W_i[t].
+    Returns dL/dx: xbar[t] = c Re sum_{q<N/2} A[q] e^{2 pi i q t/N} with
+    A[q] = sum_i w_i(q) sum_t G_i[t] e^{-2 pi i q t/N} (complex, without the
+    real part, for complex input)."""
+    G = np.asarray(G, np.complex128)
+    n = G.shape[1]
+    N = 1 << find2power(n)
+    A = np.zeros(N // 2, np.complex128)
+    for i, w in effective_windows(sc, fb, N, threads, avx):
+        gp = np.zeros(N, np.complex128)
+        gp[:n] = G[i]
+        A += w * np.fft.fft(gp)[: N // 2]
+    full = np.zeros(N, np.complex128)
+    full[: N // 2] = A
+    y = np.fft.ifft(full)[:n] * N
+    if normalize:
+        y = y / N
+    return y if complex_input else y.real
+
+
 def brute_force_cwt(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bool = True,
                     normalize: bool = True) -> np.ndarray:
     """Direct O(fn * n * N/2) evaluation for tiny inputs (no FFT anywhere).

@@ -346,3 +346,65 @@ def test_planted_mistakes_are_caught(monkeypatch):
     with monkeypatch.context() as m:
         m.setattr(np.fft, "ifft", lambda a: real_ifft(a) * a.size)  # normalisation dropped
         assert not _brute_force_matches()
+
+
+# ------------------------------------------------------------------ adjoint
+def _cwt_matrix(n, sc, fb, complex_input):
+    """The forward as an explicit (fn*n) x n matrix, from unit inputs."""
+    cols = []
+    for t in range(n):
+        e = np.zeros(n, np.complex64 if complex_input else np.float32)
+        e[t] = 1
+        cols.append(O.cwt_fp64(e, sc, fb).reshape(-1))
+    return np.stack(cols, axis=1)
+
+
+@pytest.mark.parametrize("complex_input", [False, True])
+def test_adjoint_is_the_transpose(complex_input):
+    """Brute force on a tiny case: xbar = (Re of) A^H G with A the explicit
+    forward matrix; plus the dot-product identity Re<W(x), G> = Re<x, xbar>."""
+    rng = np.random.default_rng(21)
+    n, fn, fb = 50, 7, 2.0
+    sc = O.scales(O.LOGSCALES, 100, 0.5, 45, fn)  # includes zero-coverage (stale) rows
+    A = _cwt_matrix(n, sc, fb, complex_input)
+    G = (rng.standard_normal((fn, n)) + 1j * rng.standard_normal((fn, n)))
+    ref = A.conj().T @ G.reshape(-1)
+    got = O.adjoint_fp64(G, sc, fb, complex_input=complex_input)
+    np.testing.assert_allclose(got, ref if complex_input else ref.real, rtol=0, atol=1e-12 * np.abs(ref).max())
+    x = rng.standard_normal(n) + (1j * rng.standard_normal(n) if complex_input else 0)
+    W = O.cwt_fp64(x.astype(np.complex64 if complex_input else np.float32), sc, fb)
+    x32 = x.astype(np.complex64 if complex_input else np.float32).astype(np.complex128)
+    lhs = np.real(np.vdot(W.reshape(-1), G.reshape(-1)))
+    rhs = np.real(np.vdot(x32, got))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_adjoint_matches_finite_differences():
+    """L = sum w |W|^2: central differences of L (float64) at a few samples
+    equal the adjoint of G = 2 w W."""
+    rng = np.random.default_rng(5)
+    n, fn, fb = 300, 12, 2.0
+    sc = O.scales(O.LOGSCALES, 1000, 10, 450, fn)
+    x = rng.standard_normal(n)
+    wts = rng.uniform(0, 1, (fn, n))
+
+    # cwt_fp64 rounds its input to float32; differentiate the float64 map instead
+    def W64(v):
+        N = 1 << O.find2power(n)
+        X = np.fft.rfft(np.pad(v, (0, N - n)))[: N // 2]
+        rows = []
+        for i, w in O.effective_windows(sc, fb, N):
+            full = np.zeros(N, np.complex128)
+            full[: N // 2] = X * w
+            rows.append(np.fft.ifft(full)[:n])
+        return np.stack(rows)
+
+    L = lambda v: float((wts * np.abs(W64(v)) ** 2).sum())
+    G = 2 * wts * W64(x)
+    xbar = O.adjoint_fp64(G, sc, fb)
+    h = 1e-3
+    for t in rng.choice(n, 8, replace=False):
+        e = np.zeros(n)
+        e[t] = h
+        fd = (L(x + e) - L(x - e)) / (2 * h)
+        assert abs(fd - xbar[t]) <= 1e-6 * max(1.0, abs(fd)), t
