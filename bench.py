@@ -22,6 +22,14 @@ The CPU baseline is the oracle's float32 port (numpy float32 FFTs plus the
 reference's window multiply in C) on one host core over the first
 `--cpu-rows` scales of the same workload (all 3000 by default).
 
+Other workloads (each prints its own JSON line; not the headline metric):
+  --workload long      n = 2^22 AR(2) + 3-segment tone, Morlet sigma 4, 1024
+                       LINFREQS over [0.001, 0.5] fs (34.4 GB of output)
+  --workload batched   384 channels, n ~ U[20k, 200k], fs 30 kHz, pink noise +
+                       biphasic spikes, sigma 6, 200 LOGSCALES 10 Hz-10 kHz
+  --workload adjoint   64 x 16,384 signals, 128 LOGSCALES over [0.01, 0.45] fs:
+                       the adjoint (input gradient) against the plain forward
+
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R] [--no-cpu]
   python bench.py --copy-probe MIB     one device copy of MIB MiB (counter calibration)
@@ -51,6 +59,165 @@ def headline_signal(n=100_000, fs=1000, seed=2024):
     t = np.arange(n, dtype=np.float64) / fs
     x = np.sin(2 * np.pi * f[:, None] * t[None, :] + ph[:, None]).sum(axis=0)
     return (x + 0.2 * rng.standard_normal(n)).astype(np.float32)
+
+
+def long_signal(n=1 << 22, fs=1000, seed=2025):
+    """AR(2) (x[t] = 1.6 x[t-1] - 0.8 x[t-2] + e[t]) plus a tone whose
+    frequency is constant on three equal segments."""
+    from scipy.signal import lfilter
+
+    rng = np.random.default_rng(seed)
+    ar = lfilter([1.0], [1.0, -1.6, 0.8], rng.standard_normal(n))
+    f = np.repeat(rng.uniform(0.01, 0.4, 3) * fs, -(-n // 3))[:n]
+    phase = 2 * np.pi * np.cumsum(f) / fs
+    x = ar / ar.std() + np.sin(phase)
+    return x.astype(np.float32)
+
+
+def batched_channels(count=384, fs=30_000, seed=2026):
+    """Lengths ~ U[20k, 200k]; pink (1/f) noise plus biphasic spikes at
+    Poisson 20 Hz with amplitude 5 sigma."""
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(20_000, 200_001, count)
+    chans = []
+    for n in lengths:
+        spec = np.fft.rfft(rng.standard_normal(n))
+        f = np.arange(spec.size)
+        spec[1:] /= np.sqrt(f[1:])
+        spec[0] = 0
+        x = np.fft.irfft(spec, n)
+        x /= x.std()
+        for t in np.flatnonzero(rng.random(n) < 20.0 / fs):
+            x[t: t + 15] += 5.0 * np.sin(np.linspace(0, 2 * np.pi, 15))[: n - t]
+        chans.append(x.astype(np.float32))
+    return chans
+
+
+def adjoint_signals(B=64, n=16_384, fn=128, fs=1000, seed=2024):
+    """N(0,1) plus a random linear chirp per signal; loss weights U[0,1]."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(n) / fs
+    xs = []
+    for _ in range(B):
+        f0, f1 = rng.uniform(5, 450, 2)
+        xs.append(rng.standard_normal(n) + np.sin(2 * np.pi * (f0 + (f1 - f0) * t / (2 * t[-1])) * t))
+    return np.stack(xs).astype(np.float32), rng.uniform(0, 1, (fn, n)).astype(np.float32)
+
+
+def _timed(torch, stream, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def workload_long(args):
+    import torch
+    import fcwt_amd
+
+    n, fs, fn, fb = 1 << 22, 1000, 1024, 4.0
+    x = long_signal(n, fs)
+    sc = fcwt_amd.Scales(fcwt_amd.Morlet(fb), fcwt_amd.FCWT_LINFREQS, fs, 0.001 * fs, 0.5 * fs, fn)
+    plan = fcwt_amd.Plan(n, sc.scales, fb)
+    dx = torch.from_numpy(x).cuda()
+    out = torch.empty((fn, n), dtype=torch.complex64, device="cuda")
+    st = torch.cuda.current_stream()
+    plan.profile(True)
+    ms = _timed(torch, st, lambda: plan.execute_device(dx.data_ptr(), out.data_ptr(), st.cuda_stream),
+                args.steps, args.warmup)
+    prof = plan.profile_read()
+    plan.check()
+    steps = args.steps + args.warmup
+    gbs = 8.0 * fn * n / (ms * 1e-3) / 1e9
+    return {"metric": "long-signal CWT output GB/s (n=2^22 x 1024)", "value": round(gbs, 2), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "n=4194304 AR(2)+3-segment tone, Morlet sigma=4, 1024 LINFREQS 1-500 Hz at "
+                                   "fs=1000, complex64 output 34.4 GB", "rows": plan.row_split(),
+                       "pct_hbm_peak": round(100 * gbs / HBM_PEAK_GBS, 2)},
+            "kernels": {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps} for k, v in prof.items()}}
+
+
+def workload_batched(args):
+    import torch
+    import fcwt_amd
+
+    fs, fn, fb = 30_000, 200, 6.0
+    chans = batched_channels(fs=fs)
+    morl = fcwt_amd.Morlet(fb)
+    sc = fcwt_amd.Scales(morl, fcwt_amd.FCWT_LOGSCALES, fs, 10.0, 10_000.0, fn).scales
+    plans = [fcwt_amd.Plan(c.size, sc, fb) for c in chans]
+    total = sum(c.size for c in chans)
+    xs = [torch.from_numpy(c).cuda() for c in chans]
+    out = torch.empty(fn * total, dtype=torch.complex64, device="cuda")
+    offs = np.concatenate([[0], np.cumsum([fn * c.size for c in chans])[:-1]])
+    st = torch.cuda.current_stream()
+    base = out.data_ptr()
+
+    def plain():
+        for p, x, o in zip(plans, xs, offs):
+            p.execute_device(x.data_ptr(), base + 8 * int(o), st.cuda_stream)
+
+    ms_plain = _timed(torch, st, plain, args.steps, args.warmup)
+    for p in plans:
+        p.check()
+    for p in plans:
+        p.close()
+    batch = fcwt_amd.Batch([c.size for c in chans], sc, fb)
+    x_ptrs = [x.data_ptr() for x in xs]
+    o_ptrs = [base + 8 * int(o) for o in offs]
+    ms = _timed(torch, st, lambda: batch.execute_device(x_ptrs, o_ptrs, st.cuda_stream), args.steps, args.warmup)
+    batch.check()
+    gbs = 8.0 * fn * total / (ms * 1e-3) / 1e9
+    return {"metric": "batched CWT output GB/s (384 uneven channels x 200 scales)", "value": round(gbs, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"384 channels, n in [20000, 200000] (sum {total}), fs=30000, pink noise + "
+                                   f"spikes, Morlet sigma=6, 200 LOGSCALES 10-10000 Hz, "
+                                   f"{8 * fn * total / 1e9:.1f} GB output", "mode": "one batch execute",
+                       "plans": batch.shape(), "plain_ms": round(ms_plain, 3),
+                       "batched_vs_plain_speedup": round(ms_plain / ms, 3)}}
+
+
+def workload_adjoint(args):
+    import torch
+    import fcwt_amd
+
+    B, n, fn, fs, fb = 64, 16_384, 128, 1000, 2.0
+    x, w = adjoint_signals(B, n, fn, fs)
+    sc = fcwt_amd.Scales(fcwt_amd.Morlet(fb), fcwt_amd.FCWT_LOGSCALES, fs, 0.01 * fs, 0.45 * fs, fn).scales
+    plan = fcwt_amd.Plan(n, sc, fb)
+    dx = torch.from_numpy(x).cuda()
+    W = torch.empty((B, fn, n), dtype=torch.complex64, device="cuda")
+    G = torch.randn((B, fn, n), dtype=torch.complex64, device="cuda")
+    xbar = torch.empty((B, n), dtype=torch.float32, device="cuda")
+    st = torch.cuda.current_stream()
+
+    def fwd():
+        for b in range(B):
+            plan.execute_device(dx[b].data_ptr(), W[b].data_ptr(), st.cuda_stream)
+
+    def adj():
+        for b in range(B):
+            plan.adjoint_device(G[b].data_ptr(), xbar[b].data_ptr(), st.cuda_stream)
+
+    ms_f = _timed(torch, st, fwd, args.steps, args.warmup)
+    ms_a = _timed(torch, st, adj, args.steps, args.warmup)
+    plan.check()
+    gbs = 8.0 * B * fn * n / (ms_a * 1e-3) / 1e9
+    return {"metric": "adjoint gradient-read GB/s (64 x 16384 x 128)", "value": round(gbs, 2), "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_a, 3),
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "64 signals n=16384, N(0,1)+chirp, Morlet sigma=2, 128 LOGSCALES 10-450 Hz, "
+                                   "fs=1000; adjoint reads 1.07 GB of output gradient",
+                       "forward_ms": round(ms_f, 3), "adjoint_vs_forward_speedup": round(ms_f / ms_a, 3)}}
 
 
 def cpu_baseline(x, scales, fb, rows):
@@ -225,6 +392,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--summarize-profiles", default="", metavar="DIR")
+    ap.add_argument("--workload", default="headline", choices=["headline", "long", "batched", "adjoint"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: FCWT::cwt runs on a single MI355X here "
@@ -234,6 +402,15 @@ def main():
         return
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload != "headline":
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        if "--steps" not in sys.argv:
+            args.steps = 3
+        if "--warmup" not in sys.argv:
+            args.warmup = 1
+        res = {"long": workload_long, "batched": workload_batched, "adjoint": workload_adjoint}[args.workload](args)
+        print(json.dumps(res))
         return
 
     import torch

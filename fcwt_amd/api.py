@@ -234,6 +234,62 @@ class Plan:
             pass
 
 
+class Batch:
+    """Many signals of possibly different lengths, each with its own scales,
+    transformed by one call (fcwt_hip_batch_*): signals that pad to the same
+    N run through the same launches."""
+
+    def __init__(self, lengths, scales, fb, threads=1, normalize=True, flags=0):
+        lengths = [int(v) for v in lengths]
+        if not lengths:
+            raise ValueError("Batch: no signals")
+        if isinstance(scales, np.ndarray) and scales.ndim == 1:
+            scales = [scales] * len(lengths)
+        if len(scales) != len(lengths):
+            raise ValueError("Batch: one scale vector per signal (or one for all)")
+        self._sc = [np.ascontiguousarray(s, np.float32) for s in scales]
+        self.lengths = lengths
+        self.fns = [s.size for s in self._sc]
+        cnt = len(lengths)
+        c_len = (ctypes.c_int * cnt)(*lengths)
+        c_fn = (ctypes.c_int * cnt)(*self.fns)
+        c_sc = (nat._c_float_p * cnt)(*[nat.fptr(s) for s in self._sc])
+        h = ctypes.c_void_p()
+        nat.check(nat.lib().fcwt_hip_batch_create(ctypes.byref(h), cnt, c_len, c_sc, c_fn, float(fb), int(threads),
+                                                  int(bool(normalize)), int(flags)), "fcwt_hip_batch_create")
+        self._h = h
+        self._xs = (ctypes.c_void_p * cnt)()
+        self._outs = (ctypes.c_void_p * cnt)()
+
+    def shape(self) -> dict:
+        g, t = ctypes.c_int(), ctypes.c_int()
+        nat.check(nat.lib().fcwt_hip_batch_shape(self._h, ctypes.byref(g), ctypes.byref(t)), "fcwt_hip_batch_shape")
+        return {"plans": g.value, "team_rows": t.value}
+
+    def execute_device(self, x_ptrs, out_ptrs, stream: int = 0) -> None:
+        """x_ptrs[c]: lengths[c] float32 on the device; out_ptrs[c]: fns[c] x
+        lengths[c] complex64 on the device."""
+        for c, (x, o) in enumerate(zip(x_ptrs, out_ptrs)):
+            self._xs[c] = x
+            self._outs[c] = o
+        nat.check(nat.lib().fcwt_hip_batch_execute_device(self._h, self._xs, self._outs, ctypes.c_void_p(stream)),
+                  "fcwt_hip_batch_execute_device")
+
+    def check(self) -> None:
+        nat.check(nat.lib().fcwt_hip_batch_check(self._h), "fcwt_hip_batch_check")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            nat.lib().fcwt_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class FCWT:
     """src/fcwt/fcwt.h:114-152; ``cwt`` runs on the GPU (fcwt.cpp:447-548)."""
 

@@ -54,7 +54,9 @@ struct RowD {
   int log2A;   // spreader band factor (0: direct row)
   int log2G;   // consecutive f per writer block
   int slot;    // scratch slot (two-pass rows)
-  long long out_off;
+  long long out_off;  // offset of the row in its signal's output
+  int sig;     // signal of the plan the row belongs to
+  int n;       // that signal's length (the row has n outputs)
 };
 
 struct Item {
@@ -63,19 +65,26 @@ struct Item {
 };
 
 struct KArgs {
-  const float2* spec;   // X[0, N/2)
+  const float2* spec;   // X[0, N/2) of each signal, N/2 apart
   const float* mother;  // N values
   const SegD* segs;
   const RowD* rows;
   const Item* items;
   float2* scratch;
-  float2* out;
-  int n;
+  float2* out;          // output of a one-signal plan
+  float2* const* outs;  // several signals: output of signal s at outs[s] (out unused)
   int N;
   int log2N;
   float norm;
   Twiddles tw;
 };
+
+__device__ __forceinline__ const float2* row_spec(const KArgs& a, const RowD& r) {
+  return a.spec + (size_t)r.sig * (size_t)(a.N >> 1);
+}
+__device__ __forceinline__ float2* row_out(const KArgs& a, const RowD& r) {
+  return (a.outs ? a.outs[r.sig] : a.out) + r.out_off;
+}
 
 }  // namespace fcwt_hip
 
@@ -112,7 +121,7 @@ __device__ __forceinline__ float2 spectrum_bin(const KArgs& a, const RowD& r, in
       float mx = (float)a.N - 1.0f;
       int idx = (int)(v < mx ? v : mx);
       float w = a.mother[idx];
-      float2 x = a.spec[q];
+      float2 x = row_spec(a, r)[q];
       return make_float2(x.x * w, x.y * w);
     }
   }
@@ -146,11 +155,11 @@ __global__ __launch_bounds__(kBlock) void k_writer(KArgs a) {
   }
   __syncthreads();
   lds_fft(lds, log2B, G, log2G, a.tw, 1.0f);
-  float2* out = a.out + r.out_off;
+  float2* out = row_out(a, r);
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), m = e >> log2G;
     long long t = (long long)(f0 + g) + (long long)F * m;
-    if (t < a.n) out[t] = cscale(lds[e], a.norm);
+    if (t < r.n) out[t] = cscale(lds[e], a.norm);
   }
 }
 
@@ -195,8 +204,11 @@ __global__ __launch_bounds__(kBlock) void k_spreader(KArgs a) {
 struct FwdArgs {
   const float* x;  // the input as floats (interleaved re, im for complex input)
   int n;           // floats in x per signal
+  const float* const* xs;  // if set: signal y's input at xs[y] with ns[y] * len_mul floats (x, n unused)
+  const int* ns;
+  int len_mul;
   float2* spec;    // X[0, keep) (real input: X[0, N/2))
-  float2* ztmp;    // H (four-step, real input only)
+  float2* ztmp;    // H per signal (four-step, real input only)
   float2* atmp;    // H per signal (four-step only)
   long long x_stride;     // floats between signals
   long long spec_stride;  // complex values between signals' outputs
@@ -207,6 +219,11 @@ struct FwdArgs {
   int complex_in;
   Twiddles tw;
 };
+
+__device__ __forceinline__ const float* fwd_input(const FwdArgs& a, int y) {
+  return a.xs ? a.xs[y] : a.x + y * a.x_stride;
+}
+__device__ __forceinline__ int fwd_len(const FwdArgs& a, int y) { return a.ns ? a.ns[y] * a.len_mul : a.n; }
 
 // z[j] = x[2j] + i x[2j+1] (zero past the n floats of x)
 __device__ __forceinline__ float2 packed(const float* x, int n, int j) {
@@ -227,9 +244,10 @@ __device__ __forceinline__ float2 real_post(float2 zk, float2 zc, float2 w) {
 __global__ __launch_bounds__(kBlock) void k_fwd_small(FwdArgs a) {
   __shared__ float2 lds[kLdsCap];
   const int H = 1 << a.log2H;
-  const float* x = a.x + blockIdx.y * a.x_stride;
+  const float* x = fwd_input(a, blockIdx.y);
+  const int nx = fwd_len(a, blockIdx.y);
   float2* spec = a.spec + blockIdx.y * a.spec_stride;
-  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(x, a.n, j);
+  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(x, nx, j);
   __syncthreads();
   lds_fft(lds, a.log2H, 1, 0, a.tw, -1.0f);
   if (a.complex_in) {
@@ -251,11 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
   const int G = 1 << log2G;
   const int j20 = blockIdx.x << log2G;
   const int tot = H1 << log2G;
-  const float* x = a.x + blockIdx.y * a.x_stride;
+  const float* x = fwd_input(a, blockIdx.y);
+  const int nx = fwd_len(a, blockIdx.y);
   float2* atmp = a.atmp + ((size_t)blockIdx.y << a.log2H);
   for (int e = threadIdx.x; e < tot; e += kBlock) {
     int g = e & (G - 1), j1 = e >> log2G;
-    lds[e] = packed(x, a.n, j20 + g + H2 * j1);
+    lds[e] = packed(x, nx, j20 + g + H2 * j1);
   }
   __syncthreads();
   lds_fft(lds, a.log2H1, G, log2G, a.tw, -1.0f);
@@ -288,7 +307,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
     int g = e & (G - 1), k2 = e >> log2G;
     const size_t k = (size_t)(k10 + g) + (size_t)H1 * k2;
     if (!a.complex_in)
-      a.ztmp[k] = lds[e];
+      a.ztmp[((size_t)blockIdx.y << a.log2H) + k] = lds[e];
     else if (k < (size_t)a.keep)
       spec[k] = lds[e];
   }
@@ -297,7 +316,8 @@ __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
 __global__ void k_fwd_post(FwdArgs a) {
   const int H = 1 << a.log2H;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < H) a.spec[k] = real_post(a.ztmp[k], a.ztmp[(H - k) & (H - 1)], a.tw.get((unsigned)k, -1.0f));
+  const float2* z = a.ztmp + ((size_t)blockIdx.y << a.log2H);
+  if (k < H) a.spec[blockIdx.y * a.spec_stride + k] = real_post(z[k], z[(H - k) & (H - 1)], a.tw.get((unsigned)k, -1.0f));
 }
 
 // ---- adjoint (input gradient) kernels
@@ -401,7 +421,14 @@ static int fail(int code, const std::string& msg) {
 // -------------------------------------------------------------------- plan
 struct fcwt_hip_plan_s {
   int device = 0;
-  int n = 0, N = 1, log2N = 0, fn = 0, threads = 1;
+  int n = 0, N = 1, log2N = 0, fn = 0, threads = 1;  // n: signal 0's length; fn: rows of all signals
+  // signals of the plan (all padded to the same N): length, rows, first row
+  int nsig = 1;
+  std::vector<int> sig_n, sig_fn, sig_row0;
+  int* d_sig_n = nullptr;
+  const float** d_xs = nullptr;   // per-signal input pointers (several signals)
+  float2** d_outs = nullptr;      // per-signal output pointers
+  std::vector<const void*> up_xs, up_outs;  // the pointer arrays last uploaded
   float fb = 0.f;
   bool normalize = true;
   unsigned flags = 0;
@@ -525,7 +552,8 @@ static void free_plan(fcwt_hip_plan p) {
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
                   p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
-                  p->d_adj_P, p->d_adj_atmp, p->d_adj_A, p->d_adj_y, p->d_grad, p->d_xbar};
+                  p->d_adj_P, p->d_adj_atmp, p->d_adj_A, p->d_adj_y, p->d_grad, p->d_xbar,
+                  p->d_sig_n, p->d_xs, p->d_outs};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -557,14 +585,23 @@ static Twiddles twiddles_of(const fcwt_hip_plan p) {
 
 static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
   const bool avx = !(p->flags & FCWT_HIP_COVERAGE_SCALAR);
-  std::vector<RowBand> bands =
-      build_bands(p->N, p->scales.data(), p->fn, p->threads, avx, mother_host.data(), kStaleFloor);
+  // each signal's rows, with that signal's reuse of stale bins (one
+  // reference cwt call per signal)
+  std::vector<RowBand> bands;
+  for (int sgi = 0; sgi < p->nsig; ++sgi) {
+    std::vector<RowBand> b = build_bands(p->N, p->scales.data() + p->sig_row0[sgi], p->sig_fn[sgi], p->threads, avx,
+                                         mother_host.data(), kStaleFloor);
+    bands.insert(bands.end(), b.begin(), b.end());
+  }
   p->rows.resize(p->fn);
   std::vector<int> wide, team;
   const int team_log2F = p->log2N - kTeamLog2B;
   const bool team_ok = p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F;
-  for (int i = 0; i < p->fn; ++i) {
+  for (int i = 0, sgi = 0; i < p->fn; ++i) {
+    while (i >= p->sig_row0[sgi] + p->sig_fn[sgi]) ++sgi;
     RowD& r = p->rows[i];
+    r.sig = sgi;
+    r.n = p->sig_n[sgi];
     r.seg_begin = (int)p->segs.size();
     r.seg_count = (int)bands[i].segs.size();
     for (const Segment& s : bands[i].segs) p->segs.push_back({s.lo, s.hi, s.step, 0});
@@ -582,7 +619,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     int log2F = p->log2N - r.log2B;
     r.log2G = std::min(log2F, 14 - r.log2B);
     r.slot = -1;
-    r.out_off = (long long)i * p->n;
+    r.out_off = (long long)(i - p->sig_row0[sgi]) * r.n;
   }
   if (!team.empty()) {
     // one team per XCD; rows dealt round-robin
@@ -782,6 +819,9 @@ static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, bool complex_in,
   int rc = adjoint_workspace(p);
   if (rc != FCWT_HIP_OK) return rc;
   FwdArgs f;
+  f.xs = nullptr;
+  f.ns = nullptr;
+  f.len_mul = 1;
   f.n = 2 * n;
   f.x_stride = 2LL * n;
   f.spec_stride = N / 2;
@@ -814,11 +854,38 @@ static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, bool complex_in,
   return FCWT_HIP_OK;
 }
 
-static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2* d_out, hipStream_t st) {
+// Uploads the per-signal pointer arrays of a several-signal plan when they
+// changed since the last call (stream-ordered, so earlier launches keep
+// reading the arrays they were given).
+static int upload_pointers(fcwt_hip_plan p, const float* const* xs, float2* const* outs, hipStream_t st) {
+  std::vector<const void*> nx(xs, xs + p->nsig), no(outs, outs + p->nsig);
+  if (nx != p->up_xs) {
+    p->up_xs = nx;
+    HIP_TRY(hipMemcpyAsync(p->d_xs, p->up_xs.data(), sizeof(void*) * p->nsig, hipMemcpyHostToDevice, st));
+  }
+  if (no != p->up_outs) {
+    p->up_outs = no;
+    HIP_TRY(hipMemcpyAsync(p->d_outs, p->up_outs.data(), sizeof(void*) * p->nsig, hipMemcpyHostToDevice, st));
+  }
+  return FCWT_HIP_OK;
+}
+
+// Every kernel of one execute: xs[s] / outs[s] are signal s's device input
+// and output (arrays of nsig pointers).
+static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
+                      hipStream_t st) {
+  const bool multi = p->nsig > 1;
+  if (multi) {
+    int rc = upload_pointers(p, xs, outs, st);
+    if (rc != FCWT_HIP_OK) return rc;
+  }
   // forward transform
   FwdArgs f;
-  f.x = d_x;
+  f.x = xs[0];
   f.n = complex_in ? 2 * p->n : p->n;
+  f.xs = multi ? p->d_xs : nullptr;
+  f.ns = multi ? p->d_sig_n : nullptr;
+  f.len_mul = complex_in ? 2 : 1;
   f.spec = p->d_spec;
   f.ztmp = p->d_ztmp;
   f.atmp = p->d_atmp;
@@ -828,23 +895,24 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2
   f.tw_shift = p->log2N - f.log2H;
   f.complex_in = complex_in ? 1 : 0;
   f.x_stride = 0;
-  f.spec_stride = 0;
+  f.spec_stride = p->N / 2;
   f.keep = p->N / 2;
   f.tw = twiddles_of(p);
+  const unsigned ns = (unsigned)p->nsig;
   if (complex_in && p->N == 1) {
     // N = 1: X[0, N/2) is empty; nothing reads the spectrum
   } else if (p->N >= 2) {
     ProfScope ps(p, 0, st);
     if (f.log2H <= 14) {
-      k_fwd_small<<<1, kBlock, 0, st>>>(f);
+      k_fwd_small<<<dim3(1, ns), kBlock, 0, st>>>(f);
     } else {
       int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
       int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
-      k_fwd_a<<<ga, kBlock, 0, st>>>(f);
-      k_fwd_b<<<gb, kBlock, 0, st>>>(f);
+      k_fwd_a<<<dim3(ga, ns), kBlock, 0, st>>>(f);
+      k_fwd_b<<<dim3(gb, ns), kBlock, 0, st>>>(f);
       if (!complex_in) {
         int H = 1 << f.log2H;
-        k_fwd_post<<<(H + 255) / 256, 256, 0, st>>>(f);
+        k_fwd_post<<<dim3((H + 255) / 256, ns), 256, 0, st>>>(f);
       }
     }
   }
@@ -854,8 +922,8 @@ static int launch_all(fcwt_hip_plan p, const float* d_x, bool complex_in, float2
   a.segs = p->d_segs;
   a.rows = p->d_rows;
   a.scratch = p->d_scratch;
-  a.out = d_out;
-  a.n = p->n;
+  a.out = outs[0];
+  a.outs = multi ? p->d_outs : nullptr;
   a.N = p->N;
   a.log2N = p->log2N;
   a.norm = p->normalize ? 1.0f / (float)p->N : 1.0f;
@@ -961,37 +1029,56 @@ int fcwt_hip_describe_rows(int n, const float* sc, int fn, float fb, int threads
   return FCWT_HIP_OK;
 }
 
-int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, float fb, int threads, int normalize,
-                         unsigned flags) {
-  if (!out) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: null plan pointer");
+}  // extern "C"
+
+// A plan for nsig signals that all pad to the same N: signal s has
+// lengths[s] samples and fns[s] scales at scales[s].  One-signal plans are
+// the public fcwt_hip_plan_create; batches hold one plan per padded size.
+static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const float* const* sc, const int* fns,
+                       float fb, int threads, int normalize, unsigned flags, const char* what) {
+  if (!out) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null plan pointer");
   *out = nullptr;
-  if (n < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: signal length must be >= 1");
-  if (fn < 1 || !sc) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: need fn >= 1 scales");
-  if (threads < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: threads must be >= 1");
-  if (!std::isfinite(fb)) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: wavelet bandwidth must be finite");
-  for (int i = 0; i < fn; ++i)
-    if (!(sc[i] > 0.f) || !std::isfinite(sc[i]))
-      return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: scales must be finite and > 0");
-  if (flags & ~FCWT_HIP_COVERAGE_SCALAR) return fail(FCWT_HIP_EINVAL, "fcwt_hip_plan_create: unknown flags");
-  int log2N = find2power(n);
-  if (log2N > 28) return fail(FCWT_HIP_EUNSUPPORTED, "fcwt_hip_plan_create: signal too long");
+  if (nsig < 1 || !lengths || !sc || !fns) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
+  if (threads < 1) return fail(FCWT_HIP_EINVAL, std::string(what) + ": threads must be >= 1");
+  if (!std::isfinite(fb)) return fail(FCWT_HIP_EINVAL, std::string(what) + ": wavelet bandwidth must be finite");
+  if (flags & ~FCWT_HIP_COVERAGE_SCALAR) return fail(FCWT_HIP_EINVAL, std::string(what) + ": unknown flags");
+  long long rows = 0;
+  for (int s = 0; s < nsig; ++s) {
+    if (lengths[s] < 1) return fail(FCWT_HIP_EINVAL, std::string(what) + ": signal length must be >= 1");
+    if (fns[s] < 1 || !sc[s]) return fail(FCWT_HIP_EINVAL, std::string(what) + ": need fn >= 1 scales");
+    if (find2power(lengths[s]) != find2power(lengths[0]))
+      return fail(FCWT_HIP_EINVAL, std::string(what) + ": signals of one plan must pad to the same size");
+    for (int i = 0; i < fns[s]; ++i)
+      if (!(sc[s][i] > 0.f) || !std::isfinite(sc[s][i]))
+        return fail(FCWT_HIP_EINVAL, std::string(what) + ": scales must be finite and > 0");
+    rows += fns[s];
+  }
+  if (rows > (1LL << 30)) return fail(FCWT_HIP_EUNSUPPORTED, std::string(what) + ": too many rows");
+  int log2N = find2power(lengths[0]);
+  if (log2N > 28) return fail(FCWT_HIP_EUNSUPPORTED, std::string(what) + ": signal too long");
   int dev = 0;
   int rc = check_device(&dev);
   if (rc != FCWT_HIP_OK) return rc;
 
   fcwt_hip_plan p = new (std::nothrow) fcwt_hip_plan_s;
-  if (!p) return fail(FCWT_HIP_ENOMEM, "fcwt_hip_plan_create: out of host memory");
+  if (!p) return fail(FCWT_HIP_ENOMEM, std::string(what) + ": out of host memory");
   p->device = dev;
-  p->n = n;
+  p->nsig = nsig;
+  p->n = lengths[0];
   p->log2N = log2N;
   p->N = 1 << log2N;
-  p->fn = fn;
+  p->fn = (int)rows;
   p->fb = fb;
   p->threads = threads;
   p->normalize = normalize != 0;
   p->flags = flags;
   p->team_capable = team_capable(dev, log2N);
-  p->scales.assign(sc, sc + fn);
+  for (int s = 0; s < nsig; ++s) {
+    p->sig_row0.push_back((int)p->scales.size());
+    p->sig_n.push_back(lengths[s]);
+    p->sig_fn.push_back(fns[s]);
+    p->scales.insert(p->scales.end(), sc[s], sc[s] + fns[s]);
+  }
 
   // host copy of the mother table, used only to judge which stale bins matter
   std::vector<float> mother_host(p->N);
@@ -1000,7 +1087,7 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
 
   if (hipStreamCreate(&p->stream) != hipSuccess) {
     free_plan(p);
-    return fail(FCWT_HIP_EDEVICE, "hipStreamCreate failed");
+    return fail(FCWT_HIP_EDEVICE, std::string(what) + ": hipStreamCreate failed");
   }
   const int N = p->N;
   p->s1bits = (log2N + 1) / 2;
@@ -1008,11 +1095,16 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(dalloc(p, &p->d_mother, N));
   TRY(dalloc(p, &p->d_tw1, n1));
   TRY(dalloc(p, &p->d_tw2, n2));
-  TRY(dalloc(p, &p->d_spec, std::max(1, N / 2)));
+  TRY(dalloc(p, &p->d_spec, (size_t)nsig * std::max(1, N / 2)));
   // four-step temporaries: N/2 points for real input, N for complex input
   if (log2N > 14) {
-    TRY(dalloc(p, &p->d_ztmp, (size_t)N / 2));
-    TRY(dalloc(p, &p->d_atmp, (size_t)N));
+    TRY(dalloc(p, &p->d_ztmp, (size_t)nsig * N / 2));
+    TRY(dalloc(p, &p->d_atmp, (size_t)nsig * N));
+  }
+  if (nsig > 1) {
+    TRY(dalloc(p, &p->d_sig_n, (size_t)nsig));
+    TRY(dalloc(p, &p->d_xs, (size_t)nsig));
+    TRY(dalloc(p, &p->d_outs, (size_t)nsig));
   }
   TRY(dalloc(p, &p->d_segs, p->segs.size()));
   TRY(dalloc(p, &p->d_rows, p->rows.size()));
@@ -1036,6 +1128,7 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   TRY(up(p->d_segs, p->segs.data(), p->segs.size() * sizeof(SegD)));
   TRY(up(p->d_rows, p->rows.data(), p->rows.size() * sizeof(RowD)));
   TRY(up(p->d_items, p->items.data(), p->items.size() * sizeof(Item)));
+  if (nsig > 1) TRY(up(p->d_sig_n, p->sig_n.data(), sizeof(int) * nsig));
   if (p->nteams) {
     TRY(up(p->d_team_rows, p->team_rows.data(), p->team_rows.size() * sizeof(int)));
     TRY(up(p->d_team_seq, p->team_seq.data(), p->team_seq.size() * sizeof(int)));
@@ -1052,10 +1145,17 @@ int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, flo
   k_twiddles<<<(nt + 255) / 256, 256, 0, p->stream>>>(p->d_tw1, n1, p->d_tw2, n2, p->s1bits, log2N);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) {
     free_plan(p);
-    return fail(FCWT_HIP_EDEVICE, "fcwt_hip_plan_create: table kernels failed");
+    return fail(FCWT_HIP_EDEVICE, std::string(what) + ": table kernels failed");
   }
   *out = p;
   return FCWT_HIP_OK;
+}
+
+extern "C" {
+
+int fcwt_hip_plan_create(fcwt_hip_plan* out, int n, const float* sc, int fn, float fb, int threads, int normalize,
+                         unsigned flags) {
+  return create_plan(out, 1, &n, &sc, &fn, fb, threads, normalize, flags, "fcwt_hip_plan_create");
 }
 
 int fcwt_hip_plan_destroy(fcwt_hip_plan p) {
@@ -1150,7 +1250,8 @@ static int run_device(fcwt_hip_plan p, const float* d_input, bool complex_in, fl
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
-  int rc = launch_all(p, d_input, complex_in, (float2*)d_output, (hipStream_t)stream);
+  float2* o = (float2*)d_output;
+  int rc = launch_all(p, &d_input, complex_in, &o, (hipStream_t)stream);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
 }
@@ -1178,7 +1279,8 @@ static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float*
     hipError_t e = hipMemcpyAsync(p->d_x, input, sizeof(float) * in_floats, hipMemcpyHostToDevice, p->stream);
     if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("input copy failed: ") + hipGetErrorString(e));
   }
-  if (rc == FCWT_HIP_OK) rc = launch_all(p, p->d_x, complex_in, p->d_out, p->stream);
+  const float* dx = p->d_x;
+  if (rc == FCWT_HIP_OK) rc = launch_all(p, &dx, complex_in, &p->d_out, p->stream);
   if (rc == FCWT_HIP_OK) {
     hipError_t e = hipMemcpyAsync(output, p->d_out, sizeof(float2) * out_elems, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
@@ -1195,6 +1297,7 @@ int fcwt_hip_cwt(fcwt_hip_plan p, const float* input, float* output) {
 
 int fcwt_hip_adjoint_device(fcwt_hip_plan p, const float* d_grad, float* d_xbar, int complex_input, void* stream) {
   if (!p || !d_grad || !d_xbar) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint_device: null argument");
+  if (p->nsig != 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint_device: one-signal plans only");
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
@@ -1205,6 +1308,7 @@ int fcwt_hip_adjoint_device(fcwt_hip_plan p, const float* d_grad, float* d_xbar,
 
 int fcwt_hip_adjoint(fcwt_hip_plan p, const float* grad, float* xbar, int complex_input) {
   if (!p || !grad || !xbar) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint: null argument");
+  if (p->nsig != 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint: one-signal plans only");
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
@@ -1229,6 +1333,112 @@ int fcwt_hip_adjoint(fcwt_hip_plan p, const float* grad, float* xbar, int comple
 
 int fcwt_hip_ccwt(fcwt_hip_plan p, const float* input, float* output) {
   return run_host(p, input, true, output, "fcwt_hip_ccwt");
+}
+
+// ------------------------------------------------------------------ batches
+struct fcwt_hip_batch_s {
+  struct Group {
+    fcwt_hip_plan plan = nullptr;
+    std::vector<int> members;  // signals of the batch in this plan, in plan order
+  };
+  std::vector<Group> groups;
+  int count = 0;
+  std::vector<const float*> xs;  // per-group gathering buffers
+  std::vector<float2*> outs;
+};
+
+int fcwt_hip_batch_create(fcwt_hip_batch* out, int count, const int* lengths, const float* const* scales,
+                          const int* fns, float fb, int threads, int normalize, unsigned flags) {
+  if (!out) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_create: null batch pointer");
+  *out = nullptr;
+  if (count < 1 || !lengths || !scales || !fns) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_create: bad arguments");
+  for (int c = 0; c < count; ++c)
+    if (lengths[c] < 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_create: signal length must be >= 1");
+  fcwt_hip_batch b = new (std::nothrow) fcwt_hip_batch_s;
+  if (!b) return fail(FCWT_HIP_ENOMEM, "fcwt_hip_batch_create: out of host memory");
+  b->count = count;
+  std::vector<int> order(count);
+  for (int c = 0; c < count; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return find2power(lengths[a]) < find2power(lengths[c]); });
+  for (size_t k = 0; k < order.size();) {
+    size_t e = k;
+    while (e < order.size() && find2power(lengths[order[e]]) == find2power(lengths[order[k]])) ++e;
+    fcwt_hip_batch_s::Group g;
+    g.members.assign(order.begin() + k, order.begin() + e);
+    std::vector<int> len, fnv;
+    std::vector<const float*> sc;
+    for (int c : g.members) {
+      len.push_back(lengths[c]);
+      fnv.push_back(fns[c]);
+      sc.push_back(scales[c]);
+    }
+    int rc = create_plan(&g.plan, (int)g.members.size(), len.data(), sc.data(), fnv.data(), fb, threads, normalize,
+                         flags, "fcwt_hip_batch_create");
+    if (rc != FCWT_HIP_OK) {
+      for (auto& h : b->groups) free_plan(h.plan);
+      delete b;
+      return rc;
+    }
+    b->groups.push_back(std::move(g));
+    k = e;
+  }
+  *out = b;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_batch_destroy(fcwt_hip_batch b) {
+  if (!b) return FCWT_HIP_OK;
+  for (auto& g : b->groups) free_plan(g.plan);
+  delete b;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_batch_execute_device(fcwt_hip_batch b, const float* const* d_inputs, float* const* d_outputs,
+                                  void* stream) {
+  if (!b || !d_inputs || !d_outputs) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_execute_device: null argument");
+  for (int c = 0; c < b->count; ++c)
+    if (!d_inputs[c] || !d_outputs[c]) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_execute_device: null pointer");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  int rc = FCWT_HIP_OK;
+  for (auto& g : b->groups) {
+    if (cur != g.plan->device) HIP_TRY(hipSetDevice(g.plan->device));
+    b->xs.clear();
+    b->outs.clear();
+    for (int c : g.members) {
+      b->xs.push_back(d_inputs[c]);
+      b->outs.push_back((float2*)d_outputs[c]);
+    }
+    rc = launch_all(g.plan, b->xs.data(), false, b->outs.data(), (hipStream_t)stream);
+    if (rc != FCWT_HIP_OK) break;
+  }
+  (void)hipSetDevice(cur);
+  return rc;
+}
+
+int fcwt_hip_batch_check(fcwt_hip_batch b) {
+  if (!b) return fail(FCWT_HIP_EINVAL, "null batch");
+  for (auto& g : b->groups) {
+    int rc = fcwt_hip_plan_check(g.plan);
+    if (rc != FCWT_HIP_OK) return rc;
+  }
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_batch_shape(fcwt_hip_batch b, int* groups, int* team_rows) {
+  if (!b) return fail(FCWT_HIP_EINVAL, "null batch");
+  if (groups) *groups = (int)b->groups.size();
+  if (team_rows) {
+    int t = 0;
+    for (auto& g : b->groups) {
+      int tr = 0;
+      fcwt_hip_plan_row_split(g.plan, nullptr, &tr, nullptr);
+      t += tr;
+    }
+    *team_rows = t;
+  }
+  return FCWT_HIP_OK;
 }
 
 }  // extern "C"

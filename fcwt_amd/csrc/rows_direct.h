@@ -81,7 +81,7 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   const int l0 = it.block * G;
   const int tid = threadIdx.x;
   {
-    RowSpectrum Z{a.spec, a.mother, a.segs, r.seg_begin, r.seg_count, (float)a.N - 1.0f};
+    RowSpectrum Z{row_spec(a, r), a.mother, a.segs, r.seg_begin, r.seg_count, (float)a.N - 1.0f};
     for (int q = tid; q < M; q += NT) {
       zs[q] = Z(q);
       if (P2 > 1) {
@@ -147,12 +147,12 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
     }
   }
   __syncthreads();
-  float2* out = a.out + r.out_off;
+  float2* out = row_out(a, r);
   const int tot = M * G;
   for (int e = tid; e < tot; e += NT) {
     int m = e / G, gg = e - m * G;
     long long t = (long long)(l0 + gg) + ((long long)m << log2L);
-    if (t < a.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
+    if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
   }
 }
 

@@ -327,9 +327,8 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   const float* __restrict__ mother = ta.k.mother;
   const SegD* __restrict__ segs = ta.k.segs;
   const RowD* __restrict__ rows = ta.k.rows;
-  float2* __restrict__ out = ta.k.out;
   const Twiddles tw = ta.k.tw;
-  const int n = ta.k.n, N = ta.k.N;
+  const int N = ta.k.N;
   const float norm = ta.k.norm;
   const int* __restrict__ team_rows = ta.team_rows;
   const int nteams = ta.nteams, team_size = ta.team_size;
@@ -407,6 +406,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // A * Qb = 2048 values, 8 per thread, all loads issued before the
       // first is used
       const float maxidx = (float)N - 1.0f;
+      const float2* __restrict__ rspec = spec + (size_t)r.sig * (size_t)(N >> 1);
       const int zstride = team_zs_stride(log2A);
       float2* const zs = buf + team_zs_offset(log2A, log2F);
       const int total = A << log2Qb;
@@ -423,7 +423,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       float ws[kPer];
 #pragma unroll
       for (int u = 0; u < kPer; ++u) {
-        xs[u] = idx[u] >= 0 ? spec[qs[u]] : make_float2(0.f, 0.f);
+        xs[u] = idx[u] >= 0 ? rspec[qs[u]] : make_float2(0.f, 0.f);
         ws[u] = idx[u] >= 0 ? mother[idx[u]] : 0.f;
       }
 #pragma unroll
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
       if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error)) return;
-      team_pass2<LOG2F>((gfloat2*)(out + r.out_off), n, f0, U, base);
+      team_pass2<LOG2F>((gfloat2*)row_out(ta.k, r), r.n, f0, U, base);
       // every read of this buffer has landed in registers (they fed the LDS
       // exchange), so it may be reused; the output stores need not complete
       __syncthreads();

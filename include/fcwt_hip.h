@@ -184,6 +184,31 @@ int fcwt_hip_adjoint_device(fcwt_hip_plan plan, const float* d_grad, float* d_xb
 /* The same with host pointers; returns when xbar is complete. */
 int fcwt_hip_adjoint(fcwt_hip_plan plan, const float* grad, float* xbar, int complex_input);
 
+/* ---------------------------------------------------------------------- */
+/* Batches of signals (no reference counterpart: the reference transforms one
+ * signal per FCWT::cwt call).  Signal c has lengths[c] samples and its own
+ * fns[c] scales at scales[c]; the Morlet bandwidth, threads, normalisation
+ * and flags are common.  Signals that pad to the same N share one plan, so an
+ * execute makes one forward-FFT launch and one launch of each row kernel per
+ * distinct N for all of their rows together. */
+typedef struct fcwt_hip_batch_s* fcwt_hip_batch;
+
+int fcwt_hip_batch_create(fcwt_hip_batch* batch, int count, const int* lengths, const float* const* scales,
+                          const int* fns, float fb, int threads, int normalize, unsigned flags);
+
+int fcwt_hip_batch_destroy(fcwt_hip_batch batch);
+
+/* d_inputs[c]: lengths[c] float32 samples, d_outputs[c]: fns[c]*lengths[c]
+ * complex64 (host arrays of `count` device pointers), enqueued on `stream`. */
+int fcwt_hip_batch_execute_device(fcwt_hip_batch batch, const float* const* d_inputs, float* const* d_outputs,
+                                  void* stream);
+
+/* As fcwt_hip_plan_check, for every plan of the batch. */
+int fcwt_hip_batch_check(fcwt_hip_batch batch);
+
+/* Plans (distinct padded sizes) in the batch and rows on the XCD-team kernel. */
+int fcwt_hip_batch_shape(fcwt_hip_batch batch, int* groups, int* team_rows);
+
 #ifdef __cplusplus
 }
 #endif
