@@ -204,6 +204,12 @@ class Plan:
         nat.check(fn(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(out_ptr), ctypes.c_void_p(stream)),
                   "fcwt_hip_ccwt_device" if complex_input else "fcwt_hip_cwt_device")
 
+    def spectrum_device(self, x_ptr: int, spec_ptr: int, stream: int = 0, complex_input: bool = False) -> None:
+        """The forward transform alone: X[0, N/2) complex64 of the zero-padded input."""
+        nat.check(nat.lib().fcwt_hip_spectrum_device(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(spec_ptr),
+                                                     int(bool(complex_input)), ctypes.c_void_p(stream)),
+                  "fcwt_hip_spectrum_device")
+
     def adjoint_device(self, g_ptr: int, xbar_ptr: int, stream: int = 0, complex_input: bool = False) -> None:
         """Input gradient from the output gradient (device pointers): g holds
         fn*n complex64 (dL/dRe W + i dL/dIm W), xbar receives n float32 (or n
@@ -211,6 +217,15 @@ class Plan:
         nat.check(nat.lib().fcwt_hip_adjoint_device(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(xbar_ptr),
                                                     int(bool(complex_input)), ctypes.c_void_p(stream)),
                   "fcwt_hip_adjoint_device")
+
+    def adjoint_batch_device(self, g_ptr: int, xbar_ptr: int, signals: int, stream: int = 0,
+                             complex_input: bool = False) -> None:
+        """``adjoint_device`` for ``signals`` signals at once: g holds
+        signals x fn x n complex64, xbar receives signals x n values."""
+        nat.check(nat.lib().fcwt_hip_adjoint_batch_device(self._h, ctypes.c_void_p(g_ptr), ctypes.c_void_p(xbar_ptr),
+                                                          int(signals), int(bool(complex_input)),
+                                                          ctypes.c_void_p(stream)),
+                  "fcwt_hip_adjoint_batch_device")
 
     def adjoint_host(self, g: np.ndarray, complex_input: bool = False) -> np.ndarray:
         """Host-array form of ``adjoint_device``: g is (fn, n) complex64."""

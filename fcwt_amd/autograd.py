@@ -24,7 +24,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .api import Plan
+from .api import Batch, Plan
 
 
 def _stream(t: torch.Tensor) -> int:
@@ -33,17 +33,24 @@ def _stream(t: torch.Tensor) -> int:
 
 class _CWTFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x: torch.Tensor, plan: Plan):
+    def forward(ctx, x: torch.Tensor, op: "CWT"):
         if not x.is_cuda:
             raise ValueError("CWT: the input must be a CUDA (HIP) tensor")
+        plan = op.plan
         cplx = x.is_complex()
         xin = x.detach().to(torch.complex64 if cplx else torch.float32).contiguous()
         batch = xin.shape[:-1]
         rows = xin.reshape(-1, plan.n)
-        out = torch.empty((rows.shape[0], plan.fn, plan.n), dtype=torch.complex64, device=x.device)
+        B = rows.shape[0]
+        out = torch.empty((B, plan.fn, plan.n), dtype=torch.complex64, device=x.device)
         st = _stream(x)
-        for b in range(rows.shape[0]):
-            plan.execute_device(rows[b].data_ptr(), out[b].data_ptr(), st, complex_input=cplx)
+        if B > 1 and not cplx:
+            # one set of launches for the whole batch
+            op.batch_for(B).execute_device([rows[b].data_ptr() for b in range(B)],
+                                           [out[b].data_ptr() for b in range(B)], st)
+        else:
+            for b in range(B):
+                plan.execute_device(rows[b].data_ptr(), out[b].data_ptr(), st, complex_input=cplx)
         ctx.plan = plan
         ctx.cplx = cplx
         ctx.in_dtype = x.dtype
@@ -53,11 +60,12 @@ class _CWTFunction(torch.autograd.Function):
     def backward(ctx, grad_out: torch.Tensor):
         plan, cplx = ctx.plan, ctx.cplx
         g = grad_out.to(torch.complex64).contiguous().reshape(-1, plan.fn, plan.n)
-        xbar = torch.empty((g.shape[0], plan.n), dtype=torch.complex64 if cplx else torch.float32,
-                           device=grad_out.device)
+        B = g.shape[0]
+        xbar = torch.empty((B, plan.n), dtype=torch.complex64 if cplx else torch.float32, device=grad_out.device)
         st = _stream(grad_out)
-        for b in range(g.shape[0]):
-            plan.adjoint_device(g[b].data_ptr(), xbar[b].data_ptr(), st, complex_input=cplx)
+        for b0 in range(0, B, 65535):
+            cnt = min(65535, B - b0)
+            plan.adjoint_batch_device(g[b0].data_ptr(), xbar[b0].data_ptr(), cnt, st, complex_input=cplx)
         shape = grad_out.shape[:-2] + (plan.n,)
         return xbar.reshape(shape).to(ctx.in_dtype), None
 
@@ -68,14 +76,24 @@ class CWT(torch.nn.Module):
 
     def __init__(self, n: int, scales, fb: float = 2.0, normalize: bool = True, threads: int = 1):
         super().__init__()
-        self.plan = Plan(int(n), np.ascontiguousarray(scales, np.float32), float(fb), threads, normalize)
+        self._scales = np.ascontiguousarray(scales, np.float32)
+        self._args = (float(fb), int(threads), bool(normalize))
+        self.plan = Plan(int(n), self._scales, float(fb), threads, normalize)
+        self._batches = {}
+
+    def batch_for(self, B: int) -> Batch:
+        """The batch plan for B signals of this length (built once per B)."""
+        if B not in self._batches:
+            fb, threads, normalize = self._args
+            self._batches[B] = Batch([self.plan.n] * B, self._scales, fb, threads, normalize)
+        return self._batches[B]
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
         if x.shape[-1] != self.plan.n:
             raise ValueError(f"CWT: signal length {x.shape[-1]} != planned n = {self.plan.n}")
-        return _CWTFunction.apply(x, self.plan)
+        return _CWTFunction.apply(x, self)
 
 
-def cwt(x: torch.Tensor, plan: Plan) -> torch.Tensor:
-    """Functional form: the plan's CWT of ``x`` with autograd support."""
-    return _CWTFunction.apply(x, plan)
+def cwt(x: torch.Tensor, op: CWT) -> torch.Tensor:
+    """Functional form of ``op(x)``."""
+    return _CWTFunction.apply(x, op)

@@ -88,6 +88,7 @@ __device__ __forceinline__ float2* row_out(const KArgs& a, const RowD& r) {
 
 }  // namespace fcwt_hip
 
+#include "fft_rows.h"
 #include "rows_direct.h"
 #include "rows_team.h"
 
@@ -240,23 +241,47 @@ __device__ __forceinline__ float2 real_post(float2 zk, float2 zc, float2 w) {
   return cadd(e, cmul(w, o));
 }
 
-// Whole forward transform in one work-group (H = N/2 <= kLdsCap).
-__global__ __launch_bounds__(kBlock) void k_fwd_small(FwdArgs a) {
-  __shared__ float2 lds[kLdsCap];
-  const int H = 1 << a.log2H;
+// Whole forward transform of one signal per work-group (H <= 2^14) with the
+// radix-16 register passes of fft_rows.h; grid dimension y indexes signals.
+template <int LOG2H>
+__global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
+  __shared__ RowFftLds<LOG2H> s;
+  constexpr int H = 1 << LOG2H;
   const float* x = fwd_input(a, blockIdx.y);
   const int nx = fwd_len(a, blockIdx.y);
   float2* spec = a.spec + blockIdx.y * a.spec_stride;
-  for (int j = threadIdx.x; j < H; j += kBlock) lds[j] = packed(x, nx, j);
+  row_tw_tables<LOG2H>(s);
+  for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[j] = packed(x, nx, j);
   __syncthreads();
-  lds_fft(lds, a.log2H, 1, 0, a.tw, -1.0f);
+  row_fft<LOG2H, -1>(s);
   if (a.complex_in) {
-    for (int k = threadIdx.x; k < a.keep; k += kBlock) spec[k] = lds[k];
+    for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[k];
     return;
   }
-  for (int k = threadIdx.x; k < H; k += kBlock) {
-    float2 zk = lds[k], zc = lds[(H - k) & (H - 1)];
+  for (int k = threadIdx.x; k < H; k += kRowFftThreads) {
+    float2 zk = s.data[k], zc = s.data[(H - k) & (H - 1)];
     spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
+  }
+}
+
+static void launch_rows_fft(const FwdArgs& f, unsigned batch, hipStream_t st) {
+  const dim3 g(1, batch);
+  switch (f.log2H) {
+    case 0: k_rows_fft<0><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 1: k_rows_fft<1><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 2: k_rows_fft<2><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 3: k_rows_fft<3><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 4: k_rows_fft<4><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 5: k_rows_fft<5><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 6: k_rows_fft<6><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 7: k_rows_fft<7><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 8: k_rows_fft<8><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 9: k_rows_fft<9><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 10: k_rows_fft<10><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 11: k_rows_fft<11><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 12: k_rows_fft<12><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 13: k_rows_fft<13><<<g, kRowFftThreads, 0, st>>>(f); break;
+    default: k_rows_fft<14><<<g, kRowFftThreads, 0, st>>>(f); break;
   }
 }
 
@@ -336,38 +361,43 @@ __device__ __forceinline__ float row_window(const SegD* segs, const RowD& r, int
   return 0.f;
 }
 
-// A[q] (+)= sum over rows [row0, row0 + count) of w_i(q) Ghat_i[q], q < N/2,
-// with Ghat_i = P[i - row0][0, N/2); on the last chunk A is stored conjugated,
+// For signal s = blockIdx.y of a batch (fn rows each, rows numbered s*fn + i
+// overall): A_s[q] (+)= sum of w_i(q) Ghat[s*fn + i][q] over the rows i of s
+// that lie in this chunk of rows [row0, row0 + count), q < N/2, with
+// Ghat[r] = P[r - row0].  A_s is stored conjugated once its last row is in,
 // ready for the final forward transform.
-__global__ void k_adj_reduce(const float2* P, int row0, int count, const RowD* rows, const SegD* segs,
-                             const float* mother, int N, float2* A, int first, int last) {
+__global__ void k_adj_reduce(const float2* P, long long row0, int count, int fn, const RowD* rows, const SegD* segs,
+                             const float* mother, int N, float2* A) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int half = N / 2;
-  if (q >= half) return;
+  const long long s0 = (long long)blockIdx.y * fn;
+  const long long lo = max(row0, s0), hi = min(row0 + count, s0 + fn);
+  if (q >= half || lo >= hi) return;
   const float maxidx = (float)N - 1.0f;
-  float2 acc = first ? make_float2(0.f, 0.f) : A[q];
-  for (int i = 0; i < count; ++i) {
-    const float w = row_window(segs, rows[row0 + i], q, maxidx, mother);
+  float2* a = A + (size_t)blockIdx.y * half;
+  float2 acc = lo == s0 ? make_float2(0.f, 0.f) : a[q];
+  for (long long r = lo; r < hi; ++r) {
+    const float w = row_window(segs, rows[r - s0], q, maxidx, mother);
     if (w != 0.f) {
-      const float2 g = P[(size_t)i * half + q];
+      const float2 g = P[(size_t)(r - row0) * half + q];
       acc.x += w * g.x;
       acc.y += w * g.y;
     }
   }
-  A[q] = last ? make_float2(acc.x, -acc.y) : acc;
+  a[q] = hi == s0 + fn ? make_float2(acc.x, -acc.y) : acc;
 }
 
-// xbar[t] = c Re y[t] (real input) or c conj(y[t]) (complex input), t < n,
-// where y = FFT_N(conj A): c Re / c of the inverse transform of A.
-__global__ void k_adj_out(const float2* y, int n, float c, int complex_in, float* xbar) {
+// xbar_s[t] = c Re y_s[t] (real input) or c conj(y_s[t]) (complex input),
+// t < n, s = blockIdx.y, where y_s = FFT_N(conj A_s) (N apart).
+__global__ void k_adj_out(const float2* y, int n, int N, float c, int complex_in, float* xbar) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const float2 v = y[t];
+  const float2 v = y[(size_t)blockIdx.y * N + t];
   if (complex_in) {
-    xbar[2 * t] = c * v.x;
-    xbar[2 * t + 1] = -c * v.y;
+    xbar[2 * ((size_t)blockIdx.y * n + t)] = c * v.x;
+    xbar[2 * ((size_t)blockIdx.y * n + t) + 1] = -c * v.y;
   } else {
-    xbar[t] = c * v.x;
+    xbar[(size_t)blockIdx.y * n + t] = c * v.x;
   }
 }
 
@@ -477,8 +507,9 @@ struct fcwt_hip_plan_s {
   int adj_chunk = 0;
   float2* d_adj_P = nullptr;     // adj_chunk x N/2 row spectra
   float2* d_adj_atmp = nullptr;  // adj_chunk x N four-step temporaries (N > 2^14)
-  float2* d_adj_A = nullptr;     // N/2
-  float2* d_adj_y = nullptr;     // N
+  int adj_signals = 0;           // signals the A / y buffers hold
+  float2* d_adj_A = nullptr;     // adj_signals x N/2
+  float2* d_adj_y = nullptr;     // adj_signals x N
   float* d_grad = nullptr;       // host-pointer adjoint: fn x n complex64 staging
   float* d_xbar = nullptr;
   hipStream_t stream = nullptr;
@@ -781,7 +812,7 @@ static void launch_fwd_complex(FwdArgs f, int batch, hipStream_t st) {
   f.log2H1 = f.log2H / 2;
   f.log2H2 = f.log2H - f.log2H1;
   if (f.log2H <= 14) {
-    k_fwd_small<<<dim3(1, batch), kBlock, 0, st>>>(f);
+    launch_rows_fft(f, (unsigned)batch, st);
   } else {
     int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
     int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
@@ -792,31 +823,46 @@ static void launch_fwd_complex(FwdArgs f, int batch, hipStream_t st) {
 
 constexpr size_t kAdjScratchBytes = 256u << 20;  // adjoint row-spectrum workspace
 
-static int adjoint_workspace(fcwt_hip_plan p) {
-  if (p->d_adj_y) return FCWT_HIP_OK;
+static int adjoint_workspace(fcwt_hip_plan p, int signals) {
   const size_t N = (size_t)p->N;
-  const size_t per_row = sizeof(float2) * (N / 2 + (p->log2N > 14 ? N : 0));
-  p->adj_chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)p->fn, kAdjScratchBytes / per_row));
-  int rc = dalloc(p, &p->d_adj_P, (size_t)p->adj_chunk * (N / 2));
-  if (rc == FCWT_HIP_OK && p->log2N > 14) rc = dalloc(p, &p->d_adj_atmp, (size_t)p->adj_chunk * N);
-  if (rc == FCWT_HIP_OK) rc = dalloc(p, &p->d_adj_A, N / 2);
-  if (rc == FCWT_HIP_OK) rc = dalloc(p, &p->d_adj_y, N);
-  return rc;
+  if (!p->d_adj_P) {
+    const size_t per_row = sizeof(float2) * (N / 2 + (p->log2N > 14 ? N : 0));
+    const size_t rows = (size_t)p->fn * std::max(1, signals);
+    p->adj_chunk = (int)std::max<size_t>(1, std::min<size_t>(rows, kAdjScratchBytes / per_row));
+    p->adj_chunk = std::min(p->adj_chunk, 65535);  // grid dimension y of the row transforms
+    int rc = dalloc(p, &p->d_adj_P, (size_t)p->adj_chunk * (N / 2));
+    if (rc == FCWT_HIP_OK && p->log2N > 14) rc = dalloc(p, &p->d_adj_atmp, (size_t)p->adj_chunk * N);
+    if (rc != FCWT_HIP_OK) return rc;
+  }
+  if (signals > p->adj_signals) {
+    if (p->d_adj_A) (void)hipFree(p->d_adj_A);
+    if (p->d_adj_y) (void)hipFree(p->d_adj_y);
+    p->d_adj_A = nullptr;
+    p->d_adj_y = nullptr;
+    p->adj_signals = 0;
+    int rc = dalloc(p, &p->d_adj_A, (size_t)signals * (N / 2));
+    if (rc == FCWT_HIP_OK) rc = dalloc(p, &p->d_adj_y, (size_t)signals * N);
+    if (rc != FCWT_HIP_OK) return rc;
+    p->adj_signals = signals;
+  }
+  return FCWT_HIP_OK;
 }
 
-// Input gradient from the output gradient d_grad (fn rows of n complex64):
-// per chunk of rows, Ghat_i = FFT_N(G_i)[0, N/2) and A += w_i Ghat_i; then
-// xbar = c Re IFFT_N(A)[0, n) (or the complex value for complex input).
-static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, bool complex_in, float* d_xbar, hipStream_t st) {
-  const int n = p->n, N = p->N;
+// Input gradients of `signals` signals from their output gradients d_grad
+// (signals x fn rows of n complex64, contiguous): per chunk of rows,
+// Ghat_r = FFT_N(G_r)[0, N/2) and A_s += w_i Ghat_r; then xbar_s =
+// c Re IFFT_N(A_s)[0, n) (the complex value for complex input), n apart.
+static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, int signals, bool complex_in, float* d_xbar,
+                          hipStream_t st) {
+  const int n = p->n, N = p->N, fn = p->fn;
   const float c = p->normalize ? 1.0f / (float)N : 1.0f;
+  const size_t xbar_words = (size_t)(complex_in ? 2 : 1) * n * signals;
   if (N < 2) {  // no spectrum bin: the gradient is zero
-    k_fill_u32<<<(unsigned)((2 * n + 255) / 256), 256, 0, st>>>((unsigned*)d_xbar, (size_t)(complex_in ? 2 : 1) * n,
-                                                               0u);
+    k_fill_u32<<<(unsigned)((xbar_words + 255) / 256), 256, 0, st>>>((unsigned*)d_xbar, xbar_words, 0u);
     HIP_TRY(hipGetLastError());
     return FCWT_HIP_OK;
   }
-  int rc = adjoint_workspace(p);
+  int rc = adjoint_workspace(p, signals);
   if (rc != FCWT_HIP_OK) return rc;
   FwdArgs f;
   f.xs = nullptr;
@@ -832,24 +878,31 @@ static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, bool complex_in,
   f.tw_shift = 0;
   f.tw = twiddles_of(p);
   ProfScope ps(p, 5, st);
-  for (int row0 = 0; row0 < p->fn; row0 += p->adj_chunk) {
-    const int count = std::min(p->adj_chunk, p->fn - row0);
+  const long long rows = (long long)fn * signals;
+  for (long long row0 = 0; row0 < rows; row0 += p->adj_chunk) {
+    const int count = (int)std::min<long long>(p->adj_chunk, rows - row0);
     f.x = d_grad + (size_t)row0 * 2 * n;
     f.spec = p->d_adj_P;
     launch_fwd_complex(f, count, st);
-    k_adj_reduce<<<(N / 2 + 255) / 256, 256, 0, st>>>(p->d_adj_P, row0, count, p->d_rows, p->d_segs, p->d_mother, N,
-                                                      p->d_adj_A, row0 == 0, row0 + count >= p->fn);
+    k_adj_reduce<<<dim3((N / 2 + 255) / 256, signals), 256, 0, st>>>(p->d_adj_P, row0, count, fn, p->d_rows,
+                                                                      p->d_segs, p->d_mother, N, p->d_adj_A);
   }
-  // y = FFT_N(conj A), the first n outputs
+  // y_s = FFT_N(conj A_s), the first n outputs
   f.x = (const float*)p->d_adj_A;
   f.n = N;  // N/2 complex values
-  f.x_stride = 0;
+  f.x_stride = N;
   f.spec = p->d_adj_y;
-  f.spec_stride = 0;
+  f.spec_stride = N;
   f.keep = n;
   f.atmp = p->d_adj_atmp;
-  launch_fwd_complex(f, 1, st);
-  k_adj_out<<<(n + 255) / 256, 256, 0, st>>>(p->d_adj_y, n, c, complex_in ? 1 : 0, d_xbar);
+  for (int s0 = 0; s0 < signals; s0 += p->adj_chunk) {
+    const int cnt = std::min(p->adj_chunk, signals - s0);
+    FwdArgs g = f;
+    g.x = f.x + (size_t)s0 * N;
+    g.spec = f.spec + (size_t)s0 * N;
+    launch_fwd_complex(g, cnt, st);
+  }
+  k_adj_out<<<dim3((n + 255) / 256, signals), 256, 0, st>>>(p->d_adj_y, n, N, c, complex_in ? 1 : 0, d_xbar);
   HIP_TRY(hipGetLastError());
   return FCWT_HIP_OK;
 }
@@ -872,21 +925,16 @@ static int upload_pointers(fcwt_hip_plan p, const float* const* xs, float2* cons
 
 // Every kernel of one execute: xs[s] / outs[s] are signal s's device input
 // and output (arrays of nsig pointers).
-static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
-                      hipStream_t st) {
+// The forward transform of every signal into spec (N/2 values per signal).
+static void launch_forward(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* spec, hipStream_t st) {
   const bool multi = p->nsig > 1;
-  if (multi) {
-    int rc = upload_pointers(p, xs, outs, st);
-    if (rc != FCWT_HIP_OK) return rc;
-  }
-  // forward transform
   FwdArgs f;
   f.x = xs[0];
   f.n = complex_in ? 2 * p->n : p->n;
   f.xs = multi ? p->d_xs : nullptr;
   f.ns = multi ? p->d_sig_n : nullptr;
   f.len_mul = complex_in ? 2 : 1;
-  f.spec = p->d_spec;
+  f.spec = spec;
   f.ztmp = p->d_ztmp;
   f.atmp = p->d_atmp;
   f.log2H = complex_in ? p->log2N : p->log2N - 1;
@@ -904,7 +952,7 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
   } else if (p->N >= 2) {
     ProfScope ps(p, 0, st);
     if (f.log2H <= 14) {
-      k_fwd_small<<<dim3(1, ns), kBlock, 0, st>>>(f);
+      launch_rows_fft(f, ns, st);
     } else {
       int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
       int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
@@ -916,6 +964,16 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
       }
     }
   }
+}
+
+static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
+                      hipStream_t st) {
+  const bool multi = p->nsig > 1;
+  if (multi) {
+    int rc = upload_pointers(p, xs, outs, st);
+    if (rc != FCWT_HIP_OK) return rc;
+  }
+  launch_forward(p, xs, complex_in, p->d_spec, st);
   KArgs a;
   a.spec = p->d_spec;
   a.mother = p->d_mother;
@@ -1264,6 +1322,19 @@ int fcwt_hip_ccwt_device(fcwt_hip_plan p, const float* d_input, float* d_output,
   return run_device(p, d_input, true, d_output, stream, "fcwt_hip_ccwt_device");
 }
 
+int fcwt_hip_spectrum_device(fcwt_hip_plan p, const float* d_input, float* d_spec, int complex_input, void* stream) {
+  if (!p || !d_input || !d_spec) return fail(FCWT_HIP_EINVAL, "fcwt_hip_spectrum_device: null argument");
+  if (p->nsig != 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_spectrum_device: one-signal plans only");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  launch_forward(p, &d_input, complex_input != 0, (float2*)d_spec, (hipStream_t)stream);
+  hipError_t e = hipGetLastError();
+  if (cur != p->device) (void)hipSetDevice(cur);
+  if (e != hipSuccess) return fail(FCWT_HIP_EDEVICE, std::string("fcwt_hip_spectrum_device: ") + hipGetErrorString(e));
+  return FCWT_HIP_OK;
+}
+
 // Host-pointer execute: input copy, kernels, output copy, error-word check.
 static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float* output, const char* what) {
   if (!p || !input || !output) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
@@ -1301,7 +1372,21 @@ int fcwt_hip_adjoint_device(fcwt_hip_plan p, const float* d_grad, float* d_xbar,
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
   if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
-  int rc = launch_adjoint(p, d_grad, complex_input != 0, d_xbar, (hipStream_t)stream);
+  int rc = launch_adjoint(p, d_grad, 1, complex_input != 0, d_xbar, (hipStream_t)stream);
+  if (cur != p->device) (void)hipSetDevice(cur);
+  return rc;
+}
+
+int fcwt_hip_adjoint_batch_device(fcwt_hip_plan p, const float* d_grad, float* d_xbar, int signals, int complex_input,
+                                  void* stream) {
+  if (!p || !d_grad || !d_xbar || signals < 1)
+    return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint_batch_device: bad arguments");
+  if (p->nsig != 1) return fail(FCWT_HIP_EINVAL, "fcwt_hip_adjoint_batch_device: one-signal plans only");
+  if (signals > 65535) return fail(FCWT_HIP_EUNSUPPORTED, "fcwt_hip_adjoint_batch_device: at most 65535 signals");
+  int cur = 0;
+  HIP_TRY(hipGetDevice(&cur));
+  if (cur != p->device) HIP_TRY(hipSetDevice(p->device));
+  int rc = launch_adjoint(p, d_grad, signals, complex_input != 0, d_xbar, (hipStream_t)stream);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
 }
@@ -1321,7 +1406,7 @@ int fcwt_hip_adjoint(fcwt_hip_plan p, const float* grad, float* xbar, int comple
     hipError_t e = hipMemcpyAsync(p->d_grad, grad, sizeof(float) * g_floats, hipMemcpyHostToDevice, p->stream);
     if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("gradient copy failed: ") + hipGetErrorString(e));
   }
-  if (rc == FCWT_HIP_OK) rc = launch_adjoint(p, p->d_grad, complex_input != 0, p->d_xbar, p->stream);
+  if (rc == FCWT_HIP_OK) rc = launch_adjoint(p, p->d_grad, 1, complex_input != 0, p->d_xbar, p->stream);
   if (rc == FCWT_HIP_OK) {
     hipError_t e = hipMemcpyAsync(xbar, p->d_xbar, sizeof(float) * x_floats, hipMemcpyDeviceToHost, p->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(p->stream);

@@ -168,6 +168,13 @@ int fcwt_hip_ccwt(fcwt_hip_plan plan, const float* input, float* output);
 /* The same with device pointers, enqueued on `stream`. */
 int fcwt_hip_ccwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
 
+/* The forward transform alone: X[k] = sum_{t<n} x[t] e^{-2 pi i k t/N} for
+ * k < N/2 of the zero-padded input (real, or complex with complex_input),
+ * written to d_spec as N/2 complex64 -- what the reference's FFTW r2c / c2c
+ * plans write to Ihat (src/fcwt/fcwt.cpp:486-500), the lower half of which
+ * is all the rows read.  Device pointers, enqueued on `stream`. */
+int fcwt_hip_spectrum_device(fcwt_hip_plan plan, const float* d_input, float* d_spec, int complex_input, void* stream);
+
 /* Adjoint of the plan's transform: the input gradient from an output
  * gradient (no reference counterpart -- the reference has no derivative; the
  * torch.autograd.Function of fcwt_amd uses this).  d_grad holds fn*n complex64,
@@ -183,6 +190,14 @@ int fcwt_hip_adjoint_device(fcwt_hip_plan plan, const float* d_grad, float* d_xb
 
 /* The same with host pointers; returns when xbar is complete. */
 int fcwt_hip_adjoint(fcwt_hip_plan plan, const float* grad, float* xbar, int complex_input);
+
+/* The adjoint for `signals` signals of the plan's length and scales at once
+ * (the backward pass of a batch): d_grad holds signals*fn*n complex64
+ * (signal s's rows at s*fn*n), d_xbar receives signals*n values (float32,
+ * or complex64 with complex_input), signal s at s*n.  All row transforms of
+ * the batch go through shared launches.  signals <= 65535. */
+int fcwt_hip_adjoint_batch_device(fcwt_hip_plan plan, const float* d_grad, float* d_xbar, int signals,
+                                  int complex_input, void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batches of signals (no reference counterpart: the reference transforms one

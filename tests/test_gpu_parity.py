@@ -165,3 +165,27 @@ def test_host_api_matches_device_api():
     assert np.array_equal(out, dev)
     freqs, out2 = fcwt_amd.cwt(x, 1000, 1, 101, 300, scaling="log")
     assert np.array_equal(out2, dev)
+
+
+@pytest.mark.parametrize("n", [1000, 8192, 16384, 100_000, 1 << 22])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_forward_spectrum_accuracy(n, cplx):
+    """The forward transform alone (fcwt_hip_spectrum_device) against the
+    float64 DFT: relative L2 well under 1e-6 at every size and path (one
+    work-group radix-16 FFT up to 2^14 points, four-step beyond).  Rows that
+    only read bins far below the spectrum's peak inherit this absolute error,
+    which is why a few low-frequency rows of test_zero_coverage_rows sit near
+    the 1e-5 bar (a float32 FFT of the reference would do the same)."""
+    import torch
+
+    x = tones_noise(n, fs=1000, fmax=310, seed=n)
+    x = x.astype(np.complex64) * np.complex64(1 - 0.5j) if cplx else x
+    p = fcwt_amd.Plan(n, np.array([2.0], np.float32), 2.0)
+    dx = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    sp = torch.empty(p.N // 2, dtype=torch.complex64, device="cuda")
+    p.spectrum_device(dx.data_ptr(), sp.data_ptr(), torch.cuda.current_stream().cuda_stream, cplx)
+    torch.cuda.synchronize()
+    got = sp.cpu().numpy()
+    ref = O.forward_fp64(x, p.N)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= 5e-7, err
