@@ -29,6 +29,9 @@ Other workloads (each prints its own JSON line; not the headline metric):
                        biphasic spikes, sigma 6, 200 LOGSCALES 10 Hz-10 kHz
   --workload adjoint   64 x 16,384 signals, 128 LOGSCALES over [0.01, 0.45] fs:
                        the adjoint (input gradient) against the plain forward
+  --workload host      the headline workload through the host-pointer call
+                       (fcwt_hip_cwt: input copy, kernels, 2.4 GB back to a
+                       numpy array), pipelined and as one plain copy
 
 Usage:
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R] [--no-cpu]
@@ -227,6 +230,33 @@ def workload_adjoint(args):
                        "batched_forward_vs_plain_speedup": round(ms_f / ms_fb, 3)}}
 
 
+def workload_host(args):
+    import fcwt_amd
+
+    n, fs, f0, f1, fn, fb = 100_000, 1000, 1.0, 450.0, 3000, 2.0
+    x = headline_signal(n, fs)
+    sc = fcwt_amd.Scales(fcwt_amd.Morlet(fb), fcwt_amd.FCWT_LOGSCALES, fs, f0, f1, fn)
+    plan = fcwt_amd.Plan(n, sc.scales, fb)
+    out = np.zeros((fn, n), np.complex64)
+    res = {}
+    for mode, env in (("pipelined", "0"), ("plain_copy", "1")):
+        os.environ["FCWT_HIP_NO_PIPELINE"] = env
+        for _ in range(args.warmup):
+            plan.execute_host(x, out)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute_host(x, out)
+        res[mode] = (time.perf_counter() - t0) / args.steps * 1e3
+    os.environ.pop("FCWT_HIP_NO_PIPELINE", None)
+    gbs = 8.0 * fn * n / (res["pipelined"] * 1e-3) / 1e9
+    return {"metric": "host-pointer CWT output GB/s (100k x 3000, PCIe-inclusive)", "value": round(gbs, 2),
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(res["pipelined"], 3), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "the 100k x 3000 workload through fcwt_hip_cwt into a 2.4 GB numpy array",
+                       "plain_copy_ms": round(res["plain_copy"], 3),
+                       "pipelined_vs_plain_speedup": round(res["plain_copy"] / res["pipelined"], 3)}}
+
+
 def cpu_baseline(x, scales, fb, rows):
     """Oracle float32 port on one core over the first `rows` scales."""
     from oracle import oracle as O
@@ -399,7 +429,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--summarize-profiles", default="", metavar="DIR")
-    ap.add_argument("--workload", default="headline", choices=["headline", "long", "batched", "adjoint"])
+    ap.add_argument("--workload", default="headline", choices=["headline", "long", "batched", "adjoint", "host"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: FCWT::cwt runs on a single MI355X here "
@@ -416,7 +446,8 @@ def main():
             args.steps = 3
         if "--warmup" not in sys.argv:
             args.warmup = 1
-        res = {"long": workload_long, "batched": workload_batched, "adjoint": workload_adjoint}[args.workload](args)
+        res = {"long": workload_long, "batched": workload_batched, "adjoint": workload_adjoint,
+               "host": workload_host}[args.workload](args)
         print(json.dumps(res))
         return
 

@@ -60,6 +60,8 @@ SIGNATURES = {
                                                      ctypes.c_int, ctypes.c_void_p]),
     "fcwt_hip_spectrum_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                                 ctypes.c_void_p]),
+    "fcwt_hip_tune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+    "fcwt_hip_tune_load": (ctypes.c_int, [ctypes.c_char_p]),
     "fcwt_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_int_p,
                                              ctypes.POINTER(_c_float_p), _c_int_p, ctypes.c_float, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_uint]),

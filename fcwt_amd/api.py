@@ -318,23 +318,48 @@ class FCWT:
         self._plan = None
         self._key = None
 
+    def _tune_file(self, N):
+        return f"n{N}_t{self.threads}.fcwt_hip"
+
     def create_FFT_optimization_plan(self, pmaxsize, poptimizationflags):
-        """fcwt.cpp:313-374.  A GPU plan depends on the scales, so nothing is
-        precomputed here; plans are built by ``cwt`` and cached."""
+        """fcwt.cpp:313-374: for every N = 2^11 .. 2^find2power(pmaxsize),
+        tune the GPU schedule (fcwt_hip_tune) and save n{N}_t{T}.fcwt_hip in
+        the working directory, where ``cwt`` finds it when optimisation
+        schemes are on (fcwt.cpp:376-393)."""
         pmaxsize = _as_int(pmaxsize, "FCWT_create_FFT_optimization_plan")
         if not isinstance(poptimizationflags, str):
             raise TypeError("in method 'FCWT_create_FFT_optimization_plan', argument 3 of type 'std::string'")
-        if poptimizationflags not in ("FFTW_MEASURE", "FFTW_PATIENT", "FFTW_EXHAUSTIVE", "FFTW_ESTIMATE"):
+        efforts = {"FFTW_ESTIMATE": 0, "FFTW_MEASURE": 1, "FFTW_PATIENT": 2, "FFTW_EXHAUSTIVE": 2}
+        if poptimizationflags not in efforts:
             sys.stderr.write(f"Unknown flag: {poptimizationflags}\n")
             return
-        if nat.lib().fcwt_hip_find2power(pmaxsize) <= 10:
+        nt = nat.lib().fcwt_hip_find2power(pmaxsize)
+        if nt <= 10:
             sys.stderr.write("Maxsize is too small (<=1024)... please use a larger number\n")
+        for i in range(11, nt + 1):
+            N = 1 << i
+            print(f"Calculating optimal scheme for N:{N}")
+            nat.check(nat.lib().fcwt_hip_tune(N, self.threads, efforts[poptimizationflags],
+                                              self._tune_file(N).encode()), "create_FFT_optimization_plan")
+
+    def _load_FFT_optimization_plan(self, n):
+        """fcwt.cpp:376-393."""
+        if not self.use_optimalization_schemes:
+            return
+        N = 1 << nat.lib().fcwt_hip_find2power(n)
+        if N <= 1024:
+            print("Inputsize is too small (N <= 1024) to use optimization.")
+            return
+        f = self._tune_file(N)
+        if nat.lib().fcwt_hip_tune_load(f.encode()) != nat.OK:
+            print(f"WARNING: Optimization scheme '{f}' was not found, fallback to calculation without optimization.")
 
     def plan(self, n: int, scales: Scales) -> Plan:
         key = (int(n), scales.scales.tobytes(), self.wavelet.fb, self.threads, self.use_normalization)
         if self._plan is None or self._key != key:
             if self._plan is not None:
                 self._plan.close()
+            self._load_FFT_optimization_plan(n)
             self._plan = Plan(n, scales.scales, self.wavelet.fb, self.threads, self.use_normalization)
             self._key = key
         return self._plan

@@ -71,16 +71,59 @@ FCWT::~FCWT() {
   if (plan_) fcwt_hip_plan_destroy(plan_);
 }
 
-void FCWT::create_FFT_optimization_plan(int pmaxsize, int) {
-  if (find2power(pmaxsize) <= 10) std::cerr << "Maxsize is too small (<=1024)... please use a larger number" << std::endl;
+// The tuning file of padded size N and thread count T (the reference's
+// wisdom file is n{N}_t{T}.wis, fcwt.cpp:340-341).
+static std::string tune_file(int N, int threads) {
+  return "n" + std::to_string(N) + "_t" + std::to_string(threads) + ".fcwt_hip";
 }
 
+// fcwt.cpp:313-355: for every N = 2^11 .. 2^find2power(maxsize), tune and
+// write n{N}_t{T}.fcwt_hip in the working directory.  FFTW_ESTIMATE keeps
+// the defaults; FFTW_MEASURE times each candidate once; FFTW_PATIENT and
+// FFTW_EXHAUSTIVE take the best of three.
+void FCWT::create_FFT_optimization_plan(int pmaxsize, int flags) {
+  const int nt = find2power(pmaxsize);
+  if (nt <= 10) std::cerr << "Maxsize is too small (<=1024)... please use a larger number" << std::endl;
+  const int effort = (flags & FFTW_ESTIMATE) ? 0 : ((flags & (FFTW_PATIENT | FFTW_EXHAUSTIVE)) ? 2 : 1);
+  for (int i = 11; i <= nt; ++i) {
+    const int N = 1 << i;
+    std::cout << "Calculating optimal scheme for N:" << N << std::endl;
+    const std::string f = tune_file(N, threads);
+    check(fcwt_hip_tune(N, threads, effort, f.c_str()), "FCWT::create_FFT_optimization_plan");
+    std::cout << "Optimization schemes for N: " << N << " have been calculated and saved in " << f << "." << std::endl;
+  }
+}
+
+// fcwt.cpp:356-374
 void FCWT::create_FFT_optimization_plan(int pmaxsize, std::string flags) {
-  if (flags != "FFTW_MEASURE" && flags != "FFTW_PATIENT" && flags != "FFTW_EXHAUSTIVE" && flags != "FFTW_ESTIMATE") {
+  int flag = 0;
+  if (flags == "FFTW_MEASURE") {
+    flag = FFTW_MEASURE;
+  } else if (flags == "FFTW_PATIENT") {
+    flag = FFTW_PATIENT;
+  } else if (flags == "FFTW_EXHAUSTIVE") {
+    flag = FFTW_EXHAUSTIVE;
+  } else if (flags == "FFTW_ESTIMATE") {
+    flag = FFTW_ESTIMATE;
+  } else {
     std::cerr << "Unknown flag: " << flags << std::endl;
     return;
   }
-  create_FFT_optimization_plan(pmaxsize, 0);
+  create_FFT_optimization_plan(pmaxsize, flag);
+}
+
+// fcwt.cpp:376-393: with optimisation schemes on, read the size's tuning file.
+void FCWT::load_FFT_optimization_plan(int n) {
+  if (!use_optimalization_schemes) return;
+  const int N = 1 << find2power(n);
+  if (N <= 1024) {
+    std::cout << "Inputsize is too small (N <= 1024) to use optimization." << std::endl;
+    return;
+  }
+  const std::string f = tune_file(N, threads);
+  if (fcwt_hip_tune_load(f.c_str()) != FCWT_HIP_OK)
+    std::cout << "WARNING: Optimization scheme '" << f << "' was not found, fallback to calculation without optimization."
+              << std::endl;
 }
 
 fcwt_hip_plan FCWT::plan_for(int n, Scales* sc) {
@@ -91,6 +134,7 @@ fcwt_hip_plan FCWT::plan_for(int n, Scales* sc) {
   if (plan_ && plan_n_ == n && plan_key_ == key) return plan_;
   if (plan_) fcwt_hip_plan_destroy(plan_);
   plan_ = nullptr;
+  load_FFT_optimization_plan(n);
   check(fcwt_hip_plan_create(&plan_, n, sc->scales, sc->nscales, morl->fb, threads, use_normalization ? 1 : 0, 0),
         "FCWT::cwt");
   plan_n_ = n;

@@ -24,11 +24,17 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
 #include <new>
+#include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fcwt_hip.h"
@@ -437,6 +443,19 @@ using namespace fcwt_hip;
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_last_error;
 
+// Tuned schedule parameters per padded size N (fcwt_hip_tune_load), and the
+// tuner's own override while it times candidates.
+static std::mutex g_tune_mu;
+static std::map<int, int> g_tuned_buffers;  // N -> team scratch buffers
+static thread_local int t_force_buffers = 0;
+
+static int tuned_buffers(int N) {
+  if (t_force_buffers) return t_force_buffers;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tuned_buffers.find(N);
+  return it == g_tuned_buffers.end() ? 0 : it->second;
+}
+
 static int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
@@ -512,6 +531,10 @@ struct fcwt_hip_plan_s {
   float2* d_adj_y = nullptr;     // adj_signals x N
   float* d_grad = nullptr;       // host-pointer adjoint: fn x n complex64 staging
   float* d_xbar = nullptr;
+  // host-pointer output pipeline: two pinned staging buffers and their events
+  char* h_stage[2] = {nullptr, nullptr};
+  size_t stage_bytes = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   hipStream_t stream = nullptr;
   size_t bytes = 0;
   // per-kernel timing (fcwt_hip_plan_profile): event pairs around launches
@@ -580,6 +603,10 @@ static void free_plan(fcwt_hip_plan p) {
   (void)hipSetDevice(p->device);
   drain_profile(p);
   for (hipEvent_t e : p->event_pool) (void)hipEventDestroy(e);
+  for (int k = 0; k < 2; ++k) {
+    if (p->h_stage[k]) (void)hipHostFree(p->h_stage[k]);
+    if (p->stage_ev[k]) (void)hipEventDestroy(p->stage_ev[k]);
+  }
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
                   p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
@@ -658,9 +685,11 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     p->nteams = 8;
     p->team_log2F = team_log2F;
     // scratch buffers per team: as many N-point buffers as fit the per-team
-    // scratch budget (the XCD's L2 holds them); FCWT_HIP_TEAM_BUFFERS overrides
+    // scratch budget (the XCD's L2 holds them), unless tuned
+    // (fcwt_hip_tune_load); FCWT_HIP_TEAM_BUFFERS overrides both
     const size_t row_bytes = sizeof(float2) * (size_t)p->N;
     p->team_nbuf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+    if (const int tb = tuned_buffers(p->N)) p->team_nbuf = tb;
     if (const char* e = std::getenv("FCWT_HIP_TEAM_BUFFERS")) {
       int v = std::atoi(e);
       if (v >= 2 && v <= kTeamBuffers) p->team_nbuf = v;
@@ -1336,6 +1365,66 @@ int fcwt_hip_spectrum_device(fcwt_hip_plan p, const float* d_input, float* d_spe
 }
 
 // Host-pointer execute: input copy, kernels, output copy, error-word check.
+constexpr size_t kStageBytes = 64u << 20;  // per pinned staging buffer
+
+// Copies `bytes` from src to dst with up to `threads` host threads.
+static void parallel_memcpy(char* dst, const char* src, size_t bytes, int threads) {
+  if (threads <= 1 || bytes < (4u << 20)) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> pool;
+  const size_t part = (bytes + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    const size_t lo = part * t, hi = std::min(bytes, lo + part);
+    if (lo >= hi) break;
+    pool.emplace_back([=] { std::memcpy(dst + lo, src + lo, hi - lo); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+// Device output -> caller's (pageable) host buffer in row blocks through two
+// pinned staging buffers: the copy engine fills one while host threads empty
+// the other.  FCWT_HIP_NO_PIPELINE=1 makes one plain hipMemcpy instead.
+static int copy_out(fcwt_hip_plan p, float* output, size_t bytes) {
+  const char* env = std::getenv("FCWT_HIP_NO_PIPELINE");
+  if ((env && env[0] == '1') || bytes <= kStageBytes) {
+    HIP_TRY(hipMemcpyAsync(output, p->d_out, bytes, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return FCWT_HIP_OK;
+  }
+  if (!p->h_stage[0]) {
+    for (int k = 0; k < 2; ++k) {
+      if (hipHostMalloc((void**)&p->h_stage[k], kStageBytes, hipHostMallocDefault) != hipSuccess)
+        return fail(FCWT_HIP_ENOMEM, "pinned staging allocation failed");
+      HIP_TRY(hipEventCreateWithFlags(&p->stage_ev[k], hipEventDisableTiming));
+    }
+    p->stage_bytes = kStageBytes;
+  }
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int threads = (int)std::max(1u, std::min(8u, hw ? hw : 1u));
+  const char* src = (const char*)p->d_out;
+  char* dst = (char*)output;
+  const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+  auto issue = [&](size_t c) -> int {
+    const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    HIP_TRY(hipMemcpyAsync(p->h_stage[c & 1], src + off, len, hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipEventRecord(p->stage_ev[c & 1], p->stream));
+    return FCWT_HIP_OK;
+  };
+  int rc = issue(0);
+  for (size_t c = 0; rc == FCWT_HIP_OK && c < chunks; ++c) {
+    // the buffer chunk c + 1 lands in was emptied by the host in step c - 1
+    if (c + 1 < chunks) rc = issue(c + 1);
+    if (rc != FCWT_HIP_OK) break;
+    if (hipEventSynchronize(p->stage_ev[c & 1]) != hipSuccess) return fail(FCWT_HIP_EDEVICE, "output copy failed");
+    const size_t off = c * kStageBytes, len = std::min(kStageBytes, bytes - off);
+    parallel_memcpy(dst + off, p->h_stage[c & 1], len, threads);
+  }
+  if (rc == FCWT_HIP_OK) HIP_TRY(hipStreamSynchronize(p->stream));
+  return rc;
+}
+
 static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float* output, const char* what) {
   if (!p || !input || !output) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
   int cur = 0;
@@ -1352,11 +1441,7 @@ static int run_host(fcwt_hip_plan p, const float* input, bool complex_in, float*
   }
   const float* dx = p->d_x;
   if (rc == FCWT_HIP_OK) rc = launch_all(p, &dx, complex_in, &p->d_out, p->stream);
-  if (rc == FCWT_HIP_OK) {
-    hipError_t e = hipMemcpyAsync(output, p->d_out, sizeof(float2) * out_elems, hipMemcpyDeviceToHost, p->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
-    if (e != hipSuccess) rc = fail(FCWT_HIP_EDEVICE, std::string("cwt failed: ") + hipGetErrorString(e));
-  }
+  if (rc == FCWT_HIP_OK) rc = copy_out(p, output, sizeof(float2) * out_elems);
   if (rc == FCWT_HIP_OK && p->nteams) rc = read_error_word(p);
   if (cur != p->device) (void)hipSetDevice(cur);
   return rc;
@@ -1508,6 +1593,109 @@ int fcwt_hip_batch_check(fcwt_hip_batch b) {
     int rc = fcwt_hip_plan_check(g.plan);
     if (rc != FCWT_HIP_OK) return rc;
   }
+  return FCWT_HIP_OK;
+}
+
+// ------------------------------------------------------------- tuning
+// Times one plan of n = N samples (the first N - N/4) and 1000 LOGSCALES
+// 1-450 Hz at fs = 1000 with `buffers` team scratch buffers; ms per execute.
+static int tune_time(int N, int threads, int buffers, int reps, float* ms) {
+  const int n = std::max(1, N - N / 4), fn = 1000;
+  std::vector<float> sc(fn), x(n);
+  scales(1, 1000, 1.0f, 450.0f, fn, sc.data());
+  for (int t = 0; t < n; ++t) x[t] = std::sin(0.05f * t) + 0.5f * std::sin(0.9f * t + 1.0f);
+  t_force_buffers = buffers;
+  fcwt_hip_plan p = nullptr;
+  int rc = fcwt_hip_plan_create(&p, n, sc.data(), fn, 2.0f, threads, 1, 0);
+  t_force_buffers = 0;
+  if (rc != FCWT_HIP_OK) return rc;
+  float* dx = nullptr;
+  float* dout = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  rc = FCWT_HIP_OK;
+  if (hipMalloc(&dx, sizeof(float) * n) != hipSuccess || hipMalloc(&dout, sizeof(float2) * (size_t)fn * n) != hipSuccess)
+    rc = fail(FCWT_HIP_ENOMEM, "fcwt_hip_tune: device allocation failed");
+  if (rc == FCWT_HIP_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail(FCWT_HIP_EDEVICE, "fcwt_hip_tune: event creation failed");
+  if (rc == FCWT_HIP_OK && hipMemcpy(dx, x.data(), sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(FCWT_HIP_EDEVICE, "fcwt_hip_tune: input copy failed");
+  float best = 1e30f;
+  for (int r = 0; rc == FCWT_HIP_OK && r <= reps; ++r) {
+    (void)hipEventRecord(e0, p->stream);
+    rc = fcwt_hip_cwt_device(p, dx, dout, p->stream);
+    (void)hipEventRecord(e1, p->stream);
+    if (rc == FCWT_HIP_OK) rc = fcwt_hip_plan_check(p);
+    float t = 0.f;
+    if (rc == FCWT_HIP_OK && hipEventElapsedTime(&t, e0, e1) == hipSuccess && r > 0) best = std::min(best, t);
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (dx) (void)hipFree(dx);
+  if (dout) (void)hipFree(dout);
+  fcwt_hip_plan_destroy(p);
+  *ms = best;
+  return rc;
+}
+
+int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
+  if (N < 2 || (N & (N - 1)) || threads < 1 || effort < 0 || effort > 2 || !path)
+    return fail(FCWT_HIP_EINVAL, "fcwt_hip_tune: bad arguments");
+  const int log2N = find2power(N);
+  const int log2F = log2N - kTeamLog2B;
+  int rc = FCWT_HIP_OK;
+  bool teams = false;
+  if (effort > 0) {  // effort 0 writes the defaults and needs no device
+    int dev = 0;
+    rc = check_device(&dev);
+    if (rc != FCWT_HIP_OK) return rc;
+    teams = team_capable(dev, log2N) && log2F >= kTeamMinLog2F && log2F <= kTeamMaxLog2F;
+  }
+  const size_t row_bytes = sizeof(float2) * (size_t)N;
+  int best_buf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+  float best_ms = -1.f;
+  if (teams && effort > 0) {
+    const int reps = effort == 1 ? 1 : 3;
+    for (int b : {2, 3, 4, 6, 8, 12}) {
+      if (b > kTeamBuffers) continue;
+      float ms = 0.f;
+      rc = tune_time(N, threads, b, reps, &ms);
+      if (rc != FCWT_HIP_OK) return rc;
+      if (best_ms < 0.f || ms < best_ms) {
+        best_ms = ms;
+        best_buf = b;
+      }
+    }
+  }
+  std::ofstream f(path);
+  if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
+  f << "fcwt_hip tuning 1\narch gfx950\nN " << N << "\nthreads " << threads << "\neffort " << effort
+    << "\nteam_buffers " << best_buf << "\ntime_ms " << best_ms << "\n";
+  if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned_buffers[N] = best_buf;
+  return FCWT_HIP_OK;
+}
+
+int fcwt_hip_tune_load(const char* path) {
+  if (!path) return fail(FCWT_HIP_EINVAL, "fcwt_hip_tune_load: null path");
+  std::ifstream f(path);
+  if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: cannot read ") + path);
+  std::string line, key, arch;
+  int N = 0, buffers = 0, version = 0;
+  if (!std::getline(f, line) || std::sscanf(line.c_str(), "fcwt_hip tuning %d", &version) != 1 || version != 1)
+    return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: not a tuning file: ") + path);
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    is >> key;
+    if (key == "arch") is >> arch;
+    else if (key == "N") is >> N;
+    else if (key == "team_buffers") is >> buffers;
+  }
+  if (arch != "gfx950") return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: tuned for another GPU: ") + path);
+  if (N < 2 || (N & (N - 1)) || buffers < 2 || buffers > kTeamBuffers)
+    return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: bad contents: ") + path);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned_buffers[N] = buffers;
   return FCWT_HIP_OK;
 }
 

@@ -28,6 +28,15 @@
 
 enum SCALETYPE { FCWT_LINSCALES, FCWT_LOGSCALES, FCWT_LINFREQS };
 
+// The planner flags create_FFT_optimization_plan(int, int) accepts, with
+// FFTW's values (the reference's fcwt.h brings them in through fftw3.h).
+#ifndef FFTW_MEASURE
+#define FFTW_MEASURE (0U)
+#define FFTW_EXHAUSTIVE (1U << 3)
+#define FFTW_PATIENT (1U << 5)
+#define FFTW_ESTIMATE (1U << 6)
+#endif
+
 // src/fcwt/fcwt.h:67-79
 class Wavelet {
  public:
@@ -93,10 +102,12 @@ class FCWT {
   FCWT& operator=(const FCWT&) = delete;
 
   // fcwt.cpp:313-374.  The reference writes FFTW wisdom files
-  // n{N}_t{T}.wis.  A GPU plan depends on the scales as well as on N, so
-  // here these calls only validate their arguments (unknown flag strings are
-  // reported on stderr, as the reference does); cwt builds its plan on first
-  // use and reuses it while n, the scales and the options are unchanged.
+  // n{N}_t{T}.wis; here, for every N = 2^11 .. 2^find2power(pmaxsize), the
+  // GPU schedule is tuned (fcwt_hip_tune) and saved as n{N}_t{T}.fcwt_hip in
+  // the working directory.  With use_optimalization_schemes, cwt loads the
+  // file of its size before building a plan (fcwt.cpp:376-393); cwt builds
+  // its plan on first use and reuses it while n, the scales and the options
+  // are unchanged.
   void create_FFT_optimization_plan(int pmaxsize, int poptimizationflags);
   void create_FFT_optimization_plan(int pmaxsize, std::string poptimizationflags);
 
@@ -110,6 +121,7 @@ class FCWT {
 
  private:
   fcwt_hip_plan plan_for(int n, Scales* scales);
+  void load_FFT_optimization_plan(int n);
   int threads;
   bool use_optimalization_schemes;
   bool use_normalization;

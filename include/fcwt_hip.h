@@ -200,6 +200,19 @@ int fcwt_hip_adjoint_batch_device(fcwt_hip_plan plan, const float* d_grad, float
                                   int complex_input, void* stream);
 
 /* ---------------------------------------------------------------------- */
+/* Tuning: the analogue of FCWT::create_FFT_optimization_plan and
+ * load_FFT_optimization_plan (src/fcwt/fcwt.cpp:313-393), which make and
+ * read FFTW wisdom files n{N}_t{T}.wis.  fcwt_hip_tune times the schedule
+ * choices of padded size N (power of two) on this GPU -- the XCD-team
+ * kernel's scratch buffer count -- and writes the best to `path` (a short
+ * text file) and to this process's table.  effort: 0 keeps the defaults
+ * (FFTW_ESTIMATE), 1 times each candidate once (FFTW_MEASURE), 2 takes the
+ * best of three (FFTW_PATIENT, FFTW_EXHAUSTIVE).  fcwt_hip_tune_load reads
+ * such a file; plans created afterwards for that N use it. */
+int fcwt_hip_tune(int N, int threads, int effort, const char* path);
+int fcwt_hip_tune_load(const char* path);
+
+/* ---------------------------------------------------------------------- */
 /* Batches of signals (no reference counterpart: the reference transforms one
  * signal per FCWT::cwt call).  Signal c has lengths[c] samples and its own
  * fns[c] scales at scales[c]; the Morlet bandwidth, threads, normalisation

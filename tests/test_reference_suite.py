@@ -102,14 +102,49 @@ def test_lowfrequency():
     assert np.isfinite(out).all()
 
 
-def test_plan(tmp_path, monkeypatch):
-    """tests/test_fcwt.py:228-250.  The reference's binding takes only a
-    string flag (its test passes 0 and fails); the GPU plan needs the scales,
-    so the call only validates its arguments and writes no wisdom file."""
+def test_plan_flags(tmp_path, monkeypatch, capsys):
+    """create_FFT_optimization_plan's argument handling (host only): the
+    SWIG binding takes a string flag, so the reference's own test, which
+    passes 0 (tests/test_fcwt.py:242), raises TypeError; an unknown string is
+    reported on stderr and nothing is written (fcwt.cpp:356-374)."""
     monkeypatch.chdir(tmp_path)
-    morl = Morlet(2.0)
-    f = FCWT(morl, 1, True, False)
-    f.create_FFT_optimization_plan(2048, "FFTW_MEASURE")
+    f = FCWT(Morlet(2.0), 1, True, False)
     with pytest.raises(TypeError):
         f.create_FFT_optimization_plan(2048, 0)
+    f.create_FFT_optimization_plan(2048, "FFTW_WHATEVER")
+    assert "Unknown flag" in capsys.readouterr().err
     assert not os.listdir(tmp_path)
+
+
+@pytest.mark.gpu
+def test_plan(tmp_path, monkeypatch, capsys):
+    """tests/test_fcwt.py:228-250 with the GPU tuning file in place of the
+    FFTW wisdom file: n{N}_t{T}.fcwt_hip for every N up to maxsize, read back
+    by cwt when optimisation schemes are on."""
+    monkeypatch.chdir(tmp_path)
+    fs, n = 64, 10000
+    x = np.sin(2 * np.pi * 3.3 * (np.arange(n) / fs), dtype="single")
+    morl = Morlet(2.0)
+    fc = FCWT(morl, 1, True, False)
+    fc.create_FFT_optimization_plan(2048, "FFTW_MEASURE")
+    assert os.path.isfile("n2048_t1.fcwt_hip")
+    fc = FCWT(morl, 8, True, False)
+    fc.create_FFT_optimization_plan(2048, "FFTW_ESTIMATE")
+    assert os.path.isfile("n2048_t8.fcwt_hip")
+    # a size the XCD-team kernel serves: every candidate is timed
+    fc.create_FFT_optimization_plan(1 << 15, "FFTW_MEASURE")
+    text = open("n32768_t8.fcwt_hip").read()
+    assert "arch gfx950" in text and "team_buffers" in text
+    # cwt with optimisation schemes reads the file (no warning) and gives the
+    # same result as without
+    scs = Scales(morl, FCWT_LOGSCALES, fs, 1, 32, 300)
+    a = np.zeros((300, 30_000), np.complex64)
+    xx = np.resize(x, 30_000)
+    capsys.readouterr()
+    fc.cwt(xx, scs, a)
+    assert "WARNING" not in capsys.readouterr().out
+    b = np.zeros_like(a)
+    FCWT(morl, 8, False, False).cwt(xx, scs, b)
+    assert np.array_equal(a, b)
+    FCWT(morl, 3, True, False).cwt(xx, scs, b)  # no file for T = 3
+    assert "WARNING: Optimization scheme 'n32768_t3.fcwt_hip' was not found" in capsys.readouterr().out
