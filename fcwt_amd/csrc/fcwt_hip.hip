@@ -46,7 +46,8 @@ namespace fcwt_hip {
 constexpr int kMaxLog2B = 10;  // writer FFT size cap: G * B <= kLdsCap with G >= 16
 constexpr double kStaleFloor = 1e-12;
 constexpr size_t kScratchBytes = size_t(256) << 20;
-constexpr int kKinds = 6;  // kernel kinds timed by fcwt_hip_plan_profile
+constexpr size_t kColsScratchBytes = size_t(128) << 20;  // column path: scratch slots per batch
+constexpr int kKinds = 8;  // kernel kinds timed by fcwt_hip_plan_profile
 
 struct SegD {
   int lo, hi;
@@ -92,11 +93,23 @@ __device__ __forceinline__ float2* row_out(const KArgs& a, const RowD& r) {
   return (a.outs ? a.outs[r.sig] : a.out) + r.out_off;
 }
 
+// Mother table entry w (fcwt.cpp:51-67): the argument in float32 exactly as
+// the reference forms it, the exponential in float64 rounded to float32.
+// k_mother fills the table with it; the column kernels evaluate it in place.
+__device__ __forceinline__ float mother_value(int w, float fb, float toradians, float norm) {
+#pragma clang fp contract(off)
+  const float kPi = 3.14159265358979323846264338327950288f;
+  float a = (2.0f * ((float)w * toradians) * fb - 2.0f * kPi * fb);
+  a = -(a * a) / 2;
+  return norm * (float)exp((double)a);
+}
+
 }  // namespace fcwt_hip
 
 #include "fft_rows.h"
 #include "rows_direct.h"
 #include "rows_team.h"
+#include "rows_big.h"
 
 namespace fcwt_hip {
 
@@ -407,17 +420,10 @@ __global__ void k_adj_out(const float2* y, int n, int N, float c, int complex_in
   }
 }
 
-// Mother table (fcwt.cpp:51-67): the argument in float32 exactly as the
-// reference forms it, the exponential in float64 rounded to float32.
+// Mother table (fcwt.cpp:51-67), one entry per thread.
 __global__ void k_mother(float* mother, int N, float fb, float toradians, float norm) {
-#pragma clang fp contract(off)
-  const float kPi = 3.14159265358979323846264338327950288f;
   int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w < N) {
-    float a = (2.0f * ((float)w * toradians) * fb - 2.0f * kPi * fb);
-    a = -(a * a) / 2;
-    mother[w] = norm * (float)exp((double)a);
-  }
+  if (w < N) mother[w] = mother_value(w, fb, toradians, norm);
 }
 
 // Twiddle tables, float64 sincospi rounded to float32.
@@ -508,6 +514,20 @@ struct fcwt_hip_plan_s {
   };
   std::vector<Batch> batches;
   int scratch_slots = 0;
+  // column path for wide bands at N = 256 F, F = 2^7 .. 2^14 (rows_big.h):
+  // batches of rows cols_rows[begin, begin + count), row k of a batch in
+  // scratch slot k
+  int cols_log2F = 0;
+  int cols_slots = 0;
+  std::vector<int> cols_rows;
+  struct ColsBatch {
+    int begin, count;
+  };
+  std::vector<ColsBatch> cols_batches;
+  int* d_cols_rows = nullptr;
+  float2* d_cols_scratch = nullptr;
+  float2* d_xt = nullptr;  // transposed half-spectra
+  float m_toradians = 0.f, m_norm = 0.f;  // mother-table parameters
   // device
   float* d_mother = nullptr;
   float2* d_tw1 = nullptr;
@@ -551,7 +571,8 @@ struct fcwt_hip_plan_s {
 
 // kernel kinds reported by fcwt_hip_plan_profile_read
 static const char* const kKindNames[kKinds] = {"forward_fft", "row_writer_direct", "row_spreader",
-                                               "row_writer_two_pass", "row_team", "adjoint"};
+                                               "row_writer_two_pass", "row_team", "adjoint",
+                                               "row_columns", "row_writer_columns"};
 
 static hipEvent_t take_event(fcwt_hip_plan p) {
   if (!p->event_pool.empty()) {
@@ -611,7 +632,7 @@ static void free_plan(fcwt_hip_plan p) {
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
                   p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
                   p->d_adj_P, p->d_adj_atmp, p->d_adj_A, p->d_adj_y, p->d_grad, p->d_xbar,
-                  p->d_sig_n, p->d_xs, p->d_outs};
+                  p->d_sig_n, p->d_xs, p->d_outs, p->d_cols_rows, p->d_cols_scratch, p->d_xt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -641,6 +662,13 @@ static Twiddles twiddles_of(const fcwt_hip_plan p) {
   return t;
 }
 
+// FCWT_HIP_NO_COLS=1 in the environment routes the column path's rows to
+// the generic two-pass kernels (used by tests of those).
+static bool cols_disabled() {
+  const char* e = std::getenv("FCWT_HIP_NO_COLS");
+  return e && e[0] == '1';
+}
+
 static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
   const bool avx = !(p->flags & FCWT_HIP_COVERAGE_SCALAR);
   // each signal's rows, with that signal's reuse of stale bins (one
@@ -652,9 +680,10 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     bands.insert(bands.end(), b.begin(), b.end());
   }
   p->rows.resize(p->fn);
-  std::vector<int> wide, team;
+  std::vector<int> wide, team, cols;
   const int team_log2F = p->log2N - kTeamLog2B;
   const bool team_ok = p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F;
+  const bool cols_ok = team_log2F >= kColsMinLog2F && team_log2F <= kColsMaxLog2F && !cols_disabled();
   for (int i = 0, sgi = 0; i < p->fn; ++i) {
     while (i >= p->sig_row0[sgi] + p->sig_fn[sgi]) ++sgi;
     RowD& r = p->rows[i];
@@ -669,6 +698,10 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       r.log2B = kTeamLog2B;
       r.log2A = log2M - kTeamLog2B;
       team.push_back(i);
+    } else if (log2M > kMaxLog2B && cols_ok && r.seg_count <= kTeamSegLds) {
+      r.log2B = kTeamLog2B;
+      r.log2A = log2M - kTeamLog2B;
+      cols.push_back(i);
     } else {
       r.log2B = std::min(log2M, kMaxLog2B);
       r.log2A = log2M - r.log2B;
@@ -717,6 +750,24 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
         if (st >= lag) p->team_seq[(size_t)t * p->seq_stride + len++] = 2 * (st - lag) + 1;
       }
       p->team_blocks[t] = len;
+    }
+  }
+  if (!cols.empty()) {
+    // batches of rows, one N-point scratch slot per row; the slots of one
+    // batch are meant to stay in the memory-side cache between the passes
+    p->cols_log2F = team_log2F;
+    const size_t per_row = sizeof(float2) * (size_t)p->N;
+    size_t budget = kColsScratchBytes;
+    if (const char* e = std::getenv("FCWT_HIP_COLS_SCRATCH_MB")) {
+      long v = std::atol(e);
+      if (v >= 1 && v <= 4096) budget = (size_t)v << 20;
+    }
+    p->cols_slots = (int)std::max<size_t>(1, std::min<size_t>(cols.size(), budget / per_row));
+    p->cols_rows = cols;
+    for (size_t b0 = 0; b0 < cols.size(); b0 += p->cols_slots) {
+      const int cnt = (int)std::min<size_t>(cols.size() - b0, p->cols_slots);
+      p->cols_batches.push_back({(int)b0, cnt});
+      for (int k = 0; k < cnt; ++k) p->rows[cols[b0 + k]].slot = k;
     }
   }
   // direct items, grouped by band size (one kernel instance per size)
@@ -995,6 +1046,29 @@ static void launch_forward(fcwt_hip_plan p, const float* const* xs, bool complex
   }
 }
 
+// The column path's two kernels for F = 2^LF over `count` rows.
+template <int LF>
+static void launch_cols_pass(int pass, const BigArgs& b, int count, hipStream_t st) {
+  using C = ColsCfg<LF>;
+  if (pass == 0)
+    k_cols<LF><<<dim3(256 >> C::LOG2QC, count), C::T, 0, st>>>(b);
+  else
+    k_cols_out<LF><<<dim3((1 << LF) / 16, count), 256, 0, st>>>(b);
+}
+
+static void launch_cols(int log2F, int pass, const BigArgs& b, int count, hipStream_t st) {
+  switch (log2F) {
+    case 7: launch_cols_pass<7>(pass, b, count, st); break;
+    case 8: launch_cols_pass<8>(pass, b, count, st); break;
+    case 9: launch_cols_pass<9>(pass, b, count, st); break;
+    case 10: launch_cols_pass<10>(pass, b, count, st); break;
+    case 11: launch_cols_pass<11>(pass, b, count, st); break;
+    case 12: launch_cols_pass<12>(pass, b, count, st); break;
+    case 13: launch_cols_pass<13>(pass, b, count, st); break;
+    default: launch_cols_pass<14>(pass, b, count, st); break;
+  }
+}
+
 static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
                       hipStream_t st) {
   const bool multi = p->nsig > 1;
@@ -1042,6 +1116,29 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     void* args[] = {&ta};
     HIP_TRY(hipLaunchCooperativeKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args,
                                        0, st));
+  }
+  if (!p->cols_batches.empty()) {
+    const int lf = p->cols_log2F;
+    BigArgs b;
+    b.k = a;
+    b.xt = p->d_xt;
+    b.scratch = p->d_cols_scratch;
+    b.fb = p->fb;
+    b.toradians = p->m_toradians;
+    b.mnorm = p->m_norm;
+    {
+      ProfScope ps(p, 6, st);
+      k_spec_transpose<<<dim3(16 << (lf - 7), (unsigned)p->nsig), 256, 0, st>>>(p->d_spec, p->d_xt, lf);
+    }
+    for (const auto& cb : p->cols_batches) {
+      b.rows = p->d_cols_rows + cb.begin;
+      {
+        ProfScope ps(p, 6, st);
+        launch_cols(lf, 0, b, cb.count, st);
+      }
+      ProfScope ps(p, 7, st);
+      launch_cols(lf, 1, b, cb.count, st);
+    }
   }
   for (const auto& bt : p->batches) {
     {
@@ -1197,6 +1294,11 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   TRY(dalloc(p, &p->d_rows, p->rows.size()));
   TRY(dalloc(p, &p->d_items, p->items.size()));
   if (p->scratch_slots) TRY(dalloc(p, &p->d_scratch, (size_t)p->scratch_slots * N));
+  if (p->cols_slots) {
+    TRY(dalloc(p, &p->d_cols_rows, p->cols_rows.size()));
+    TRY(dalloc(p, &p->d_cols_scratch, (size_t)p->cols_slots * N));
+    TRY(dalloc(p, &p->d_xt, (size_t)nsig * (N / 2)));
+  }
   TRY(dalloc(p, &p->d_error, 1));
   if (p->nteams) {
     TRY(dalloc(p, &p->d_team_rows, p->team_rows.size()));
@@ -1216,6 +1318,7 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   TRY(up(p->d_rows, p->rows.data(), p->rows.size() * sizeof(RowD)));
   TRY(up(p->d_items, p->items.data(), p->items.size() * sizeof(Item)));
   if (nsig > 1) TRY(up(p->d_sig_n, p->sig_n.data(), sizeof(int) * nsig));
+  if (p->cols_slots) TRY(up(p->d_cols_rows, p->cols_rows.data(), p->cols_rows.size() * sizeof(int)));
   if (p->nteams) {
     TRY(up(p->d_team_rows, p->team_rows.data(), p->team_rows.size() * sizeof(int)));
     TRY(up(p->d_team_seq, p->team_seq.data(), p->team_seq.size() * sizeof(int)));
@@ -1227,6 +1330,8 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   const float kPi = 3.14159265358979323846264338327950288f;
   float toradians = (2 * kPi) / (float)N;
   float norm = sqrtf(2 * kPi) * 0.75112554446f;
+  p->m_toradians = toradians;
+  p->m_norm = norm;
   k_mother<<<(N + 255) / 256, 256, 0, p->stream>>>(p->d_mother, N, fb, toradians, norm);
   int nt = std::max(n1, n2);
   k_twiddles<<<(nt + 255) / 256, 256, 0, p->stream>>>(p->d_tw1, n1, p->d_tw2, n2, p->s1bits, log2N);

@@ -1,0 +1,230 @@
+// rows_big.h -- band-limited inverse FFT for wide scales (band M > 1024 bins)
+// at padded sizes N = 256 F, 2^7 <= F <= 2^14, in two launches per batch of
+// rows with the intermediate in one global scratch slot per row.
+// Device code, included by fcwt_hip.hip after rows_team.h.
+//
+// With the band M = 256 A (A <= F/2):
+//   y[f + F m] = c sum_{qb<256} W_256^{qb m} W_N^{qb f} U[qb][f]
+//   U[qb][f]   = sum_{qf<A} Z[qb + 256 qf] W_F^{qf f}
+//
+// Pass 1 (k_cols): a work-group transforms Qc = W/F consecutive columns qb of
+// one row (W = 2^13 points, 2^14 when F = 2^14).  It reads each column's
+// band from the signal's transposed spectrum Xt[qb][qf] (contiguous along
+// qf), multiplies it by the window, whose value it computes with the mother
+// table's own float32 formula (a gather from the table would touch one cache
+// line per bin), runs F-point inverse FFTs as radix-16 Stockham passes in
+// LDS, and stores U[qb][0, F) of its columns as one contiguous run of the
+// row's scratch slot.
+// Pass 2 (k_cols_out): a work-group takes 16 consecutive f of one row: it
+// reads U[qb][f0, f0 + 16) (128-byte runs), applies W_N^{qb f} and the
+// 256-point inverse FFT over qb = j + 16 k as two register radix-16 steps
+// around one LDS exchange, and writes y[f0 + g + F m] as 128-byte runs.
+#pragma once
+#include "fft_lds.h"
+#include "fft_reg.h"
+
+namespace fcwt_hip {
+
+constexpr int kColsMinLog2F = 7;
+constexpr int kColsMaxLog2F = 14;
+
+struct BigArgs {
+  KArgs k;
+  const float2* xt;  // transposed half-spectra, signal s at xt + s N/2: [qb][qf], F/2 values per qb
+  const int* rows;   // the batch's rows: blockIdx.y -> row rows[blockIdx.y], scratch slot blockIdx.y
+  float2* scratch;   // slot y at scratch + y N, laid out U[qb][f]
+  float fb, toradians, mnorm;  // the mother table's parameters (k_mother)
+};
+
+// Xt[s][qb][qf] = X_s[qb + 256 qf] for qb < 256, qf < F/2, through 64 x 16
+// LDS tiles; grid (16 * ceil(F/128), signals).
+__global__ __launch_bounds__(256) void k_spec_transpose(const float2* __restrict__ spec, float2* __restrict__ xt,
+                                                        int log2F) {
+  __shared__ float2 tile[64 * 17];
+  const int Fh = 1 << (log2F - 1);
+  const int qb0 = (blockIdx.x & 15) << 4, qf0 = (blockIdx.x >> 4) << 6;
+  const size_t sig = (size_t)blockIdx.y * ((size_t)Fh << 8);
+  const float2* X = spec + sig;
+  float2* Y = xt + sig;
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int qb = e & 15, qf = e >> 4;
+    if (qf0 + qf < Fh) tile[qf * 17 + qb] = X[((size_t)(qf0 + qf) << 8) + qb0 + qb];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 1024; e += 256) {
+    const int qf = e & 63, qb = e >> 6;
+    if (qf0 + qf < Fh) Y[(size_t)(qb0 + qb) * Fh + qf0 + qf] = tile[qf * 17 + qb];
+  }
+}
+
+template <int LOG2F>
+struct ColsCfg {
+  static constexpr int LOG2W = LOG2F > 13 ? LOG2F : 13;  // points per work-group
+  static constexpr int W = 1 << LOG2W;
+  static constexpr int T = W >> 4;                        // threads: 16 points each
+  static constexpr int LOG2QC = LOG2W - LOG2F;            // columns per work-group
+  static constexpr int PADW = W + (W >> 4);
+};
+
+// LDS index of point i: one pad word per 16 points keeps the radix-16 writes
+// (stride 16) on distinct banks.
+__device__ __forceinline__ int cpad(int i) { return i + (i >> 4); }
+
+// W_F^r from the LDS tables t1[r & 127] = W_F^{r mod 128}, t2[r >> 7] = W_F^{128 (r >> 7)}
+__device__ __forceinline__ float2 cols_tw(const float2* t1, const float2* t2, int r) {
+  return cmul(t2[r >> 7], t1[r & 127]);
+}
+
+// One radix-2^LOG2R Stockham pass over the work-group's columns (each F
+// points, natural order in and out); Ns = 2^LOG2NS points already combined.
+template <int LOG2F, int LOG2R, int LOG2NS>
+__device__ __forceinline__ void cols_pass(float2* d, const float2* t1, const float2* t2) {
+  using C = ColsCfg<LOG2F>;
+  constexpr int R = 1 << LOG2R, Ns = 1 << LOG2NS;
+  constexpr int LOG2GPC = LOG2F - LOG2R;            // groups per column
+  constexpr int PER = (C::W >> LOG2R) / C::T;       // groups per thread
+  float2 v[PER][R];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int g = threadIdx.x + u * C::T;
+    const int c0 = (g >> LOG2GPC) << LOG2F, gl = g & ((1 << LOG2GPC) - 1);
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[u][k] = d[cpad(c0 + gl + (k << LOG2GPC))];
+    if constexpr (LOG2NS > 0) {
+      const int jj = gl & (Ns - 1);
+#pragma unroll
+      for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], cols_tw(t1, t2, (jj * k) << (LOG2F - LOG2NS - LOG2R)));
+    }
+    dft_reg<R, 1>(v[u]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int g = threadIdx.x + u * C::T;
+    const int c0 = (g >> LOG2GPC) << LOG2F, gl = g & ((1 << LOG2GPC) - 1);
+    const int dst = c0 + ((gl >> LOG2NS) << (LOG2NS + LOG2R)) + (gl & (Ns - 1));
+#pragma unroll
+    for (int k = 0; k < R; ++k) d[cpad(dst + (k << LOG2NS))] = v[u][k];
+  }
+  __syncthreads();
+}
+
+// In-place inverse F-point DFTs of the work-group's columns: radix
+// 2^(LOG2F mod 4) first, radix 16 after.
+template <int LOG2F>
+__device__ __forceinline__ void cols_fft(float2* d, const float2* t1, const float2* t2) {
+  constexpr int FIRST = LOG2F % 4;
+  if constexpr (FIRST > 0) cols_pass<LOG2F, FIRST, 0>(d, t1, t2);
+  cols_pass<LOG2F, 4, FIRST>(d, t1, t2);
+  if constexpr (LOG2F >= FIRST + 8) cols_pass<LOG2F, 4, FIRST + 4>(d, t1, t2);
+  if constexpr (LOG2F >= FIRST + 12) cols_pass<LOG2F, 4, FIRST + 8>(d, t1, t2);
+  static_assert(LOG2F >= kColsMinLog2F && LOG2F <= kColsMaxLog2F, "F out of range");
+}
+
+// Pass 1: grid (256 / Qc column blocks, rows in the batch).
+template <int LOG2F>
+__global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols(BigArgs a) {
+  using C = ColsCfg<LOG2F>;
+  constexpr int F = 1 << LOG2F;
+  constexpr int PT = C::W / C::T;  // points per thread: 16
+  __shared__ float2 d[C::PADW];
+  __shared__ float2 t1[128], t2[F / 128];
+  __shared__ SegD sseg[kTeamSegLds];
+  const RowD r = a.k.rows[a.rows[blockIdx.y]];
+  const int qb0 = blockIdx.x << C::LOG2QC;
+  for (int e = threadIdx.x; e < 128 + F / 128; e += C::T) {  // W_F^r = W_N^{256 r}
+    if (e < 128)
+      t1[e] = a.k.tw((unsigned)e << 8);
+    else
+      t2[e - 128] = a.k.tw((unsigned)(e - 128) << 15);
+  }
+  const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
+  if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = a.k.segs[r.seg_begin + threadIdx.x];
+  __syncthreads();
+  // the columns' windowed bands: all loads issued before the first is used
+  const int A = min(1 << r.log2A, F >> 1);  // the band lies below N/2
+  const float2* __restrict__ X = a.xt + (size_t)r.sig * (size_t)(a.k.N >> 1);
+  const float maxidx = (float)a.k.N - 1.0f;
+  float2 xs[PT];
+#pragma unroll
+  for (int u = 0; u < PT; ++u) {
+    const int e = u * C::T + threadIdx.x;
+    const int c = e >> LOG2F, qf = e & (F - 1);
+    xs[u] = qf < A ? X[(size_t)(qb0 + c) * (F >> 1) + qf] : make_float2(0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < PT; ++u) {
+    const int e = u * C::T + threadIdx.x;
+    const int c = e >> LOG2F, qf = e & (F - 1);
+    float2 z = make_float2(0.f, 0.f);
+    if (qf < A) {
+      const int idx = window_index(qb0 + c + (qf << 8), nseg, sseg, maxidx);
+      if (idx >= 0) {
+        const float w = mother_value(idx, a.fb, a.toradians, a.mnorm);
+        z = make_float2(xs[u].x * w, xs[u].y * w);
+      }
+    }
+    d[cpad(e)] = z;
+  }
+  __syncthreads();
+  cols_fft<LOG2F>(d, t1, t2);
+  gfloat2* U = (gfloat2*)(a.scratch + (size_t)blockIdx.y * a.k.N) + ((size_t)qb0 << LOG2F);
+#pragma unroll
+  for (int u = 0; u < PT; ++u) {
+    const int e = u * C::T + threadIdx.x;
+    gstore(U + e, d[cpad(e)]);
+  }
+}
+
+// Pass 2: grid (F / 16 f blocks, rows in the batch), 256 threads.
+//   thread (g, j): f = f0 + g, qb = j + 16 k (k < 16) in registers
+//   wb[k1][j] = W_256^{j k1}   t2s[g][k] = W_N^{16 k f}   base = c W_N^{j f}
+template <int LOG2F>
+__global__ __launch_bounds__(256) void k_cols_out(BigArgs a) {
+  constexpr int XS = 16 * 17;  // exchange [k1][g][j]: k1 * XS + g * 17 + j
+  __shared__ float2 xb[16 * XS];  // exchange, then the tile [m][g] (m * 17 + g)
+  __shared__ float2 wb[256], t2s[256];
+  const RowD r = a.k.rows[a.rows[blockIdx.y]];
+  const int N = a.k.N;
+  const int f0 = blockIdx.x << 4;
+  const int g = threadIdx.x & 15, j = threadIdx.x >> 4;
+  {
+    const int k1 = threadIdx.x >> 4, jj = threadIdx.x & 15;
+    wb[threadIdx.x] = a.k.tw((unsigned)(k1 * jj) << LOG2F);
+    const int gg = threadIdx.x >> 4, k = threadIdx.x & 15;
+    t2s[threadIdx.x] = a.k.tw((unsigned)((16 * k * (f0 + gg)) & (N - 1)));
+  }
+  const float2 base = cscale(a.k.tw((unsigned)(j * (f0 + g))), a.k.norm);
+  const gfloat2* U = (const gfloat2*)(a.scratch + (size_t)blockIdx.y * N);
+  float2 v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = gload(U + ((size_t)(j + 16 * k) << LOG2F) + f0 + g);
+  __syncthreads();  // tables
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], t2s[g * 16 + k]);
+  dft_reg<16, 1>(v);
+  v[0] = cmul(v[0], base);
+#pragma unroll
+  for (int k1 = 1; k1 < 16; ++k1) v[k1] = cmul(v[k1], cmul(base, wb[k1 * 16 + j]));
+#pragma unroll
+  for (int k1 = 0; k1 < 16; ++k1) xb[k1 * XS + g * 17 + j] = v[k1];
+  __syncthreads();
+  const int k1 = threadIdx.x >> 4;
+#pragma unroll
+  for (int jj = 0; jj < 16; ++jj) v[jj] = xb[k1 * XS + g * 17 + jj];
+  dft_reg<16, 1>(v);  // over j: output k2, m = k1 + 16 k2
+  __syncthreads();
+#pragma unroll
+  for (int k2 = 0; k2 < 16; ++k2) xb[(k1 + 16 * k2) * 17 + g] = v[k2];
+  __syncthreads();
+  gfloat2* out = (gfloat2*)row_out(a.k, r);
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int e = s * 256 + threadIdx.x;
+    const int m = e >> 4, gg = e & 15;
+    const long long t = (long long)(f0 + gg) + ((long long)m << LOG2F);
+    if (t < r.n) gstore_nt(out + t, xb[m * 17 + gg]);
+  }
+}
+
+}  // namespace fcwt_hip
