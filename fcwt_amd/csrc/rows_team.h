@@ -21,7 +21,8 @@
 // for its stores (s_waitcnt), then bumps its buffer's arrival counter; a
 // pass-2 item waits for the row's count and reads with device-scope loads,
 // which bypass the CU's L1 and are served by the shared L2, then bumps the
-// buffer's done counter, which the next pass 1 into that buffer waits for.
+// buffer's done counter, which the next pass 1 into that buffer waits for
+// (after staging its band in LDS, so that those loads overlap the wait).
 // nbuf buffers per team give the sequence slack on both sides.  Every wait is
 // bounded: on timeout the kernel records an error and every work-group exits.
 //
@@ -396,12 +397,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
             post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << kTeamLog2B);
           }
       }
-      // the buffer's previous row has been read by every pass-2 item
-      if (gen > 0) {
-        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error)) return;
-      } else {
-        __syncthreads();
-      }
+      __syncthreads();  // segments and tables
       // the row's windowed band on this item's bins, into LDS: at most
       // A * Qb = 2048 values, 8 per thread, all loads issued before the
       // first is used
@@ -432,7 +428,13 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
         const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;
         if (e < total) zs[ql * zstride + qf] = make_float2(xs[u].x * ws[u], xs[u].y * ws[u]);
       }
-      __syncthreads();
+      // the buffer's previous row has been read by every pass-2 item (the
+      // wait ends in a barrier, which also publishes the staged band)
+      if (gen > 0) {
+        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error)) return;
+      } else {
+        __syncthreads();
+      }
       team_pass1_for<LOG2F>(log2A, qb0, U);
       // the pass-1 stores are complete (s_waitcnt) before the arrival is counted
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
