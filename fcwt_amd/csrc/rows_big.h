@@ -110,15 +110,45 @@ __device__ __forceinline__ void cols_pass(float2* d, const float2* t1, const flo
 }
 
 // In-place inverse F-point DFTs of the work-group's columns: radix
-// 2^(LOG2F mod 4) first, radix 16 after.
+// 2^(LOG2F mod 4) first, radix 16 after, from pass `skip` on.  When a
+// column's input is zero past A <= F / S (S the product of the radices of
+// the first `skip` passes), those passes only replicate: afterwards
+// d[g S + t] = z[g] for t < S.  The caller stores its input that way and
+// skips them.
 template <int LOG2F>
-__device__ __forceinline__ void cols_fft(float2* d, const float2* t1, const float2* t2) {
+__device__ __forceinline__ void cols_fft(float2* d, const float2* t1, const float2* t2, int skip) {
   constexpr int FIRST = LOG2F % 4;
-  if constexpr (FIRST > 0) cols_pass<LOG2F, FIRST, 0>(d, t1, t2);
-  cols_pass<LOG2F, 4, FIRST>(d, t1, t2);
-  if constexpr (LOG2F >= FIRST + 8) cols_pass<LOG2F, 4, FIRST + 4>(d, t1, t2);
-  if constexpr (LOG2F >= FIRST + 12) cols_pass<LOG2F, 4, FIRST + 8>(d, t1, t2);
+  int j = 0;
+  if constexpr (FIRST > 0) {
+    if (skip <= j) cols_pass<LOG2F, FIRST, 0>(d, t1, t2);
+    ++j;
+  }
+  if (skip <= j) cols_pass<LOG2F, 4, FIRST>(d, t1, t2);
+  ++j;
+  if constexpr (LOG2F >= FIRST + 8) {
+    if (skip <= j) cols_pass<LOG2F, 4, FIRST + 4>(d, t1, t2);
+    ++j;
+  }
+  if constexpr (LOG2F >= FIRST + 12) {
+    if (skip <= j) cols_pass<LOG2F, 4, FIRST + 8>(d, t1, t2);
+  }
   static_assert(LOG2F >= kColsMinLog2F && LOG2F <= kColsMaxLog2F, "F out of range");
+}
+
+// Passes of cols_fft that only replicate for a band of A values, and the
+// log2 of their radix product.
+template <int LOG2F>
+__device__ __forceinline__ int cols_skip(int A, int* log2S) {
+  constexpr int FIRST = LOG2F % 4;
+  constexpr int NPASS = (FIRST > 0) + (LOG2F - FIRST) / 4;
+  int skip = 0, l2 = 0, step = FIRST > 0 ? FIRST : 4;
+  while (skip < NPASS && (A << (l2 + step)) <= (1 << LOG2F)) {
+    l2 += step;
+    ++skip;
+    step = 4;
+  }
+  *log2S = l2;
+  return skip;
 }
 
 // Pass 1: grid (256 / Qc column blocks, rows in the batch).
@@ -141,33 +171,64 @@ __global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols(BigArgs a) {
   const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
   if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = a.k.segs[r.seg_begin + threadIdx.x];
   __syncthreads();
-  // the columns' windowed bands: all loads issued before the first is used
+  // the columns' windowed bands, each value stored at its S places
+  // (cols_fft skips the passes that would only replicate it); all loads are
+  // issued before the first is used
   const int A = min(1 << r.log2A, F >> 1);  // the band lies below N/2
+  int log2S;
+  const int skip = cols_skip<LOG2F>(A, &log2S);
+  const int log2G = LOG2F - log2S;  // values per column
   const float2* __restrict__ X = a.xt + (size_t)r.sig * (size_t)(a.k.N >> 1);
   const float maxidx = (float)a.k.N - 1.0f;
   float2 xs[PT];
+  if (skip == 0) {
 #pragma unroll
-  for (int u = 0; u < PT; ++u) {
-    const int e = u * C::T + threadIdx.x;
-    const int c = e >> LOG2F, qf = e & (F - 1);
-    xs[u] = qf < A ? X[(size_t)(qb0 + c) * (F >> 1) + qf] : make_float2(0.f, 0.f);
-  }
+    for (int u = 0; u < PT; ++u) {
+      const int e = u * C::T + threadIdx.x;
+      const int c = e >> LOG2F, qf = e & (F - 1);
+      xs[u] = qf < A ? X[(size_t)(qb0 + c) * (F >> 1) + qf] : make_float2(0.f, 0.f);
+    }
 #pragma unroll
-  for (int u = 0; u < PT; ++u) {
-    const int e = u * C::T + threadIdx.x;
-    const int c = e >> LOG2F, qf = e & (F - 1);
-    float2 z = make_float2(0.f, 0.f);
-    if (qf < A) {
-      const int idx = window_index(qb0 + c + (qf << 8), nseg, sseg, maxidx);
-      if (idx >= 0) {
-        const float w = mother_value(idx, a.fb, a.toradians, a.mnorm);
-        z = make_float2(xs[u].x * w, xs[u].y * w);
+    for (int u = 0; u < PT; ++u) {
+      const int e = u * C::T + threadIdx.x;
+      const int c = e >> LOG2F, qf = e & (F - 1);
+      float2 z = make_float2(0.f, 0.f);
+      if (qf < A) {
+        const int idx = window_index(qb0 + c + (qf << 8), nseg, sseg, maxidx);
+        if (idx >= 0) {
+          const float w = mother_value(idx, a.fb, a.toradians, a.mnorm);
+          z = make_float2(xs[u].x * w, xs[u].y * w);
+        }
+      }
+      d[cpad(e)] = z;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int e = u * C::T + threadIdx.x;  // value g of column c
+      const int c = e >> log2G, g = e & ((1 << log2G) - 1);
+      xs[u] = (g < A && c < (1 << C::LOG2QC)) ? X[(size_t)(qb0 + c) * (F >> 1) + g] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+      const int e = u * C::T + threadIdx.x;
+      const int c = e >> log2G, g = e & ((1 << log2G) - 1);
+      if (c < (1 << C::LOG2QC)) {
+        float2 z = make_float2(0.f, 0.f);
+        if (g < A) {
+          const int idx = window_index(qb0 + c + (g << 8), nseg, sseg, maxidx);
+          if (idx >= 0) {
+            const float w = mother_value(idx, a.fb, a.toradians, a.mnorm);
+            z = make_float2(xs[u].x * w, xs[u].y * w);
+          }
+        }
+        const int d0 = (c << LOG2F) + (g << log2S);
+        for (int t = 0; t < (1 << log2S); ++t) d[cpad(d0 + t)] = z;
       }
     }
-    d[cpad(e)] = z;
   }
   __syncthreads();
-  cols_fft<LOG2F>(d, t1, t2);
+  cols_fft<LOG2F>(d, t1, t2, skip);
   gfloat2* U = (gfloat2*)(a.scratch + (size_t)blockIdx.y * a.k.N) + ((size_t)qb0 << LOG2F);
 #pragma unroll
   for (int u = 0; u < PT; ++u) {
