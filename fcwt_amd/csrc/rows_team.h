@@ -46,7 +46,7 @@ constexpr int kTeamBuffers = 12;               // most scratch buffers per team
 constexpr size_t kTeamScratchPerXcd = 3u << 20;  // scratch bytes per team (default buffer count)
 constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
-constexpr int kTeamSegLds = 32;                // row segments staged in LDS
+constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths
 constexpr long kTeamSpin = 1L << 22;           // polls (with s_sleep) before a wait is declared failed
 
 struct TeamArgs {
@@ -119,6 +119,25 @@ __device__ __forceinline__ int window_index(int q, int nseg, const SegD* sseg, f
   return (int)(v < maxidx ? v : maxidx);
 }
 
+// The same with the row's segments read from global memory: the address is
+// uniform over the work-group, so these are scalar loads, and no LDS copy
+// (with its barrier) is needed.
+__device__ __forceinline__ int window_index_g(int q, int nseg, const SegD* __restrict__ gseg, float maxidx) {
+#pragma clang fp contract(off)
+  bool hit = false;
+  float step = 0.f;
+  for (int s = 0; s < nseg; ++s) {
+    const SegD sg = gseg[s];
+    if (q >= sg.lo && q < sg.hi) {
+      hit = true;
+      step = sg.step;
+    }
+  }
+  if (!hit) return -1;
+  float v = step * (float)q;
+  return (int)(v < maxidx ? v : maxidx);
+}
+
 // Pass-1 register split of the A-point transform: R1 = min(A, 16) points per
 // thread, P2 = A / R1 threads per transform.
 __host__ __device__ constexpr int team_log2r1(int log2A) { return log2A < 4 ? log2A : 4; }
@@ -153,7 +172,6 @@ __shared__ float2 tm_pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
 __shared__ float2 tm_post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
 __shared__ float2 tm_t2[kTeamB];                // pass-2 [g][k]
 __shared__ float2 tm_wb[kTeamB];                // pass-2 [k1][j]
-__shared__ SegD tm_seg[kTeamSegLds];            // the row's segments
 
 // Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A.
 //   band (tm_buf + zs_off)  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
@@ -321,7 +339,6 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   float2* const post = tm_post;
   float2* const t2s = tm_t2;
   float2* const wb = tm_wb;
-  SegD* const sseg = tm_seg;
   __shared__ int s_rank, s_flag;
   // plain copies of the arguments (never take the address of the kernel argument)
   const float2* __restrict__ spec = ta.k.spec;
@@ -379,7 +396,6 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // ---- pass 1 of row i, qb block blk
       const int qb0 = blk << log2Qb;
       const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
-      if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = segs[r.seg_begin + threadIdx.x];
       const int log2A = r.log2A, A = 1 << log2A;
       if (log2A != tab_log2A) {
         tab_log2A = log2A;
@@ -397,7 +413,6 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
             post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << kTeamLog2B);
           }
       }
-      __syncthreads();  // segments and tables
       // the row's windowed band on this item's bins, into LDS: at most
       // A * Qb = 2048 values, 8 per thread, all loads issued before the
       // first is used
@@ -413,7 +428,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
         const int e = u * kTeamThreads + threadIdx.x;
         const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
         qs[u] = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
-        idx[u] = e < total ? window_index(qs[u], nseg, sseg, maxidx) : -1;
+        idx[u] = e < total ? window_index_g(qs[u], nseg, segs + r.seg_begin, maxidx) : -1;
       }
       float2 xs[kPer];
       float ws[kPer];
