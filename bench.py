@@ -279,8 +279,8 @@ KIND_KERNELS = {
     "row_team": ("k_rows_team",),
     "row_spreader": ("k_spreader",),
     "row_writer_two_pass": ("k_writer",),
-    "row_columns": ("k_spec_transpose", "k_cols"),
-    "row_writer_columns": ("k_cols_out",),
+    "row_columns": ("k_cols_step",),
+    "column_transpose": ("k_spec_transpose",),
 }
 WORKLOAD_TAG = "100k_x_3000"
 
@@ -290,7 +290,7 @@ def kind_bytes(kind, n, N, split):
     if kind == "forward_fft":
         return 4.0 * n + 4.0 * N  # signal read + N/2 complex64 spectrum written
     rows = {"row_writer_direct": split["direct"], "row_team": split["team"],
-            "row_writer_two_pass": split["two_pass"], "row_writer_columns": split["two_pass"]}.get(kind, 0)
+            "row_writer_two_pass": split["two_pass"], "row_columns": split["two_pass"]}.get(kind, 0)
     return 8.0 * n * rows
 
 

@@ -525,7 +525,8 @@ struct fcwt_hip_plan_s {
   };
   std::vector<ColsBatch> cols_batches;
   int* d_cols_rows = nullptr;
-  float2* d_cols_scratch = nullptr;
+  int cols_sets = 1;                  // scratch sets: 2 when there are several batches
+  float2* d_cols_scratch = nullptr;   // cols_sets x cols_slots x N
   float2* d_xt = nullptr;  // transposed half-spectra
   float m_toradians = 0.f, m_norm = 0.f;  // mother-table parameters
   // device
@@ -572,7 +573,7 @@ struct fcwt_hip_plan_s {
 // kernel kinds reported by fcwt_hip_plan_profile_read
 static const char* const kKindNames[kKinds] = {"forward_fft", "row_writer_direct", "row_spreader",
                                                "row_writer_two_pass", "row_team", "adjoint",
-                                               "row_columns", "row_writer_columns"};
+                                               "row_columns", "column_transpose"};
 
 static hipEvent_t take_event(fcwt_hip_plan p) {
   if (!p->event_pool.empty()) {
@@ -769,6 +770,10 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       p->cols_batches.push_back({(int)b0, cnt});
       for (int k = 0; k < cnt; ++k) p->rows[cols[b0 + k]].slot = k;
     }
+    // with several batches and F <= 2^13, batch k uses scratch set k & 1 and
+    // pass 1 of batch k runs in the launch of pass 2 of batch k - 1 (at
+    // F = 2^14 a pass-1 work-group fills a CU's LDS, so the passes alternate)
+    p->cols_sets = (p->cols_batches.size() > 1 && team_log2F < 14) ? 2 : 1;
   }
   // direct items, grouped by band size (one kernel instance per size)
   for (int lm = 0; lm <= kMaxLog2B; ++lm) {
@@ -1046,26 +1051,29 @@ static void launch_forward(fcwt_hip_plan p, const float* const* xs, bool complex
   }
 }
 
-// The column path's two kernels for F = 2^LF over `count` rows.
+// One column-path launch for F = 2^LF: pass 1 of n1 rows (b1) beside pass 2
+// of n2 rows (b2); pass 2 alone (n1 = 0) runs as 256-thread work-groups.
 template <int LF>
-static void launch_cols_pass(int pass, const BigArgs& b, int count, hipStream_t st) {
+static void launch_cols_t(const BigArgs& b1, int n1, const BigArgs& b2, int n2, hipStream_t st) {
   using C = ColsCfg<LF>;
-  if (pass == 0)
-    k_cols<LF><<<dim3(256 >> C::LOG2QC, count), C::T, 0, st>>>(b);
-  else
-    k_cols_out<LF><<<dim3((1 << LF) / 16, count), 256, 0, st>>>(b);
+  if (n1 == 0) {
+    k_cols_out<LF><<<dim3((1 << LF) / 16, n2), 256, 0, st>>>(b2);
+    return;
+  }
+  const unsigned wgs = (unsigned)((256 >> C::LOG2QC) * n1 + ((1 << LF) / 16) / (C::T / 256) * n2);
+  k_cols_step<LF><<<wgs, C::T, 0, st>>>(b1, n1, b2, n2);
 }
 
-static void launch_cols(int log2F, int pass, const BigArgs& b, int count, hipStream_t st) {
+static void launch_cols(int log2F, const BigArgs& b1, int n1, const BigArgs& b2, int n2, hipStream_t st) {
   switch (log2F) {
-    case 7: launch_cols_pass<7>(pass, b, count, st); break;
-    case 8: launch_cols_pass<8>(pass, b, count, st); break;
-    case 9: launch_cols_pass<9>(pass, b, count, st); break;
-    case 10: launch_cols_pass<10>(pass, b, count, st); break;
-    case 11: launch_cols_pass<11>(pass, b, count, st); break;
-    case 12: launch_cols_pass<12>(pass, b, count, st); break;
-    case 13: launch_cols_pass<13>(pass, b, count, st); break;
-    default: launch_cols_pass<14>(pass, b, count, st); break;
+    case 7: launch_cols_t<7>(b1, n1, b2, n2, st); break;
+    case 8: launch_cols_t<8>(b1, n1, b2, n2, st); break;
+    case 9: launch_cols_t<9>(b1, n1, b2, n2, st); break;
+    case 10: launch_cols_t<10>(b1, n1, b2, n2, st); break;
+    case 11: launch_cols_t<11>(b1, n1, b2, n2, st); break;
+    case 12: launch_cols_t<12>(b1, n1, b2, n2, st); break;
+    case 13: launch_cols_t<13>(b1, n1, b2, n2, st); break;
+    default: launch_cols_t<14>(b1, n1, b2, n2, st); break;
   }
 }
 
@@ -1127,17 +1135,30 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     b.toradians = p->m_toradians;
     b.mnorm = p->m_norm;
     {
-      ProfScope ps(p, 6, st);
+      ProfScope ps(p, 7, st);
       k_spec_transpose<<<dim3(16 << (lf - 7), (unsigned)p->nsig), 256, 0, st>>>(p->d_spec, p->d_xt, lf);
     }
-    for (const auto& cb : p->cols_batches) {
-      b.rows = p->d_cols_rows + cb.begin;
-      {
-        ProfScope ps(p, 6, st);
-        launch_cols(lf, 0, b, cb.count, st);
+    const size_t set_elems = (size_t)p->cols_slots * p->N;
+    const size_t nb = p->cols_batches.size();
+    auto batch_args = [&](size_t k) {
+      BigArgs x = b;
+      x.rows = p->d_cols_rows + p->cols_batches[k].begin;
+      x.scratch = p->d_cols_scratch + (k & 1) * (p->cols_sets - 1) * set_elems;
+      return x;
+    };
+    ProfScope ps(p, 6, st);
+    if (p->cols_sets == 2) {
+      // launch k: pass 1 of batch k (scratch set k & 1) beside pass 2 of batch k - 1
+      for (size_t k = 0; k <= nb; ++k) {
+        const BigArgs b1 = k < nb ? batch_args(k) : b, b2 = k > 0 ? batch_args(k - 1) : b;
+        launch_cols(lf, b1, k < nb ? p->cols_batches[k].count : 0, b2, k > 0 ? p->cols_batches[k - 1].count : 0, st);
       }
-      ProfScope ps(p, 7, st);
-      launch_cols(lf, 1, b, cb.count, st);
+    } else {
+      for (size_t k = 0; k < nb; ++k) {
+        const BigArgs bk = batch_args(k);
+        launch_cols(lf, bk, p->cols_batches[k].count, b, 0, st);
+        launch_cols(lf, b, 0, bk, p->cols_batches[k].count, st);
+      }
     }
   }
   for (const auto& bt : p->batches) {
@@ -1296,7 +1317,7 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   if (p->scratch_slots) TRY(dalloc(p, &p->d_scratch, (size_t)p->scratch_slots * N));
   if (p->cols_slots) {
     TRY(dalloc(p, &p->d_cols_rows, p->cols_rows.size()));
-    TRY(dalloc(p, &p->d_cols_scratch, (size_t)p->cols_slots * N));
+    TRY(dalloc(p, &p->d_cols_scratch, (size_t)p->cols_sets * p->cols_slots * N));
     TRY(dalloc(p, &p->d_xt, (size_t)nsig * (N / 2)));
   }
   TRY(dalloc(p, &p->d_error, 1));

@@ -1,13 +1,14 @@
 // rows_big.h -- band-limited inverse FFT for wide scales (band M > 1024 bins)
-// at padded sizes N = 256 F, 2^7 <= F <= 2^14, in two launches per batch of
-// rows with the intermediate in one global scratch slot per row.
+// at padded sizes N = 256 F, 2^7 <= F <= 2^14, in two passes per batch of
+// rows with the intermediate in one global scratch slot per row; each launch
+// runs pass 1 of one batch beside pass 2 of the previous one.
 // Device code, included by fcwt_hip.hip after rows_team.h.
 //
 // With the band M = 256 A (A <= F/2):
 //   y[f + F m] = c sum_{qb<256} W_256^{qb m} W_N^{qb f} U[qb][f]
 //   U[qb][f]   = sum_{qf<A} Z[qb + 256 qf] W_F^{qf f}
 //
-// Pass 1 (k_cols): a work-group transforms Qc = W/F consecutive columns qb of
+// Pass 1: a work-group transforms Qc = W/F consecutive columns qb of
 // one row (W = 2^13 points, 2^14 when F = 2^14).  It reads each column's
 // band from the signal's transposed spectrum Xt[qb][qf] (contiguous along
 // qf), multiplies it by the window, whose value it computes with the mother
@@ -15,10 +16,10 @@
 // line per bin), runs F-point inverse FFTs as radix-16 Stockham passes in
 // LDS, and stores U[qb][0, F) of its columns as one contiguous run of the
 // row's scratch slot.
-// Pass 2 (k_cols_out): a work-group takes 16 consecutive f of one row: it
-// reads U[qb][f0, f0 + 16) (128-byte runs), applies W_N^{qb f} and the
-// 256-point inverse FFT over qb = j + 16 k as two register radix-16 steps
-// around one LDS exchange, and writes y[f0 + g + F m] as 128-byte runs.
+// Pass 2: 256 threads take 16 consecutive f of one row.  They read
+// U[qb][f0, f0 + 16) (128-byte runs), apply W_N^{qb f} and the 256-point
+// inverse FFT over qb = j + 16 k as two register radix-16 steps around one
+// LDS exchange, and write y[f0 + g + F m] as 128-byte runs.
 #pragma once
 #include "fft_lds.h"
 #include "fft_reg.h"
@@ -151,17 +152,31 @@ __device__ __forceinline__ int cols_skip(int A, int* log2S) {
   return skip;
 }
 
-// Pass 1: grid (256 / Qc column blocks, rows in the batch).
+// LDS of the column kernel (complex64 values): pass 1 holds the columns
+// (PADW), its twiddle tables (128 + F/128) and the row's segments; pass 2
+// holds, per 256-thread item, the exchange/tile (16 x 272) and two factor
+// tables of 256.
 template <int LOG2F>
-__global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols(BigArgs a) {
+struct ColsLds {
+  using C = ColsCfg<LOG2F>;
+  static constexpr int P1 = C::PADW + 128 + (1 << LOG2F) / 128 + 2 * kTeamSegLds;
+  static constexpr int P2_ITEM = 16 * 272 + 512;
+  static constexpr int P2 = (C::T / 256) * P2_ITEM;
+  static constexpr int SIZE = P1 > P2 ? P1 : P2;
+};
+
+// Pass 1 for row y of the batch, column block xblk (Qc consecutive qb).
+template <int LOG2F>
+__device__ __forceinline__ void cols_pass1(const BigArgs& a, int y, int xblk, float2* lds) {
   using C = ColsCfg<LOG2F>;
   constexpr int F = 1 << LOG2F;
   constexpr int PT = C::W / C::T;  // points per thread: 16
-  __shared__ float2 d[C::PADW];
-  __shared__ float2 t1[128], t2[F / 128];
-  __shared__ SegD sseg[kTeamSegLds];
-  const RowD r = a.k.rows[a.rows[blockIdx.y]];
-  const int qb0 = blockIdx.x << C::LOG2QC;
+  float2* const d = lds;
+  float2* const t1 = lds + C::PADW;
+  float2* const t2 = t1 + 128;
+  SegD* const sseg = reinterpret_cast<SegD*>(t2 + F / 128);
+  const RowD r = a.k.rows[a.rows[y]];
+  const int qb0 = xblk << C::LOG2QC;
   for (int e = threadIdx.x; e < 128 + F / 128; e += C::T) {  // W_F^r = W_N^{256 r}
     if (e < 128)
       t1[e] = a.k.tw((unsigned)e << 8);
@@ -229,7 +244,7 @@ __global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols(BigArgs a) {
   }
   __syncthreads();
   cols_fft<LOG2F>(d, t1, t2, skip);
-  gfloat2* U = (gfloat2*)(a.scratch + (size_t)blockIdx.y * a.k.N) + ((size_t)qb0 << LOG2F);
+  gfloat2* U = (gfloat2*)(a.scratch + (size_t)y * a.k.N) + ((size_t)qb0 << LOG2F);
 #pragma unroll
   for (int u = 0; u < PT; ++u) {
     const int e = u * C::T + threadIdx.x;
@@ -237,26 +252,29 @@ __global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols(BigArgs a) {
   }
 }
 
-// Pass 2: grid (F / 16 f blocks, rows in the batch), 256 threads.
+// Pass 2 for row y of the batch, f block fblk (16 consecutive f), by the 256
+// threads tid of one item (every thread of the work-group reaches its
+// barriers).
 //   thread (g, j): f = f0 + g, qb = j + 16 k (k < 16) in registers
 //   wb[k1][j] = W_256^{j k1}   t2s[g][k] = W_N^{16 k f}   base = c W_N^{j f}
 template <int LOG2F>
-__global__ __launch_bounds__(256) void k_cols_out(BigArgs a) {
+__device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, int tid, float2* lds) {
   constexpr int XS = 16 * 17;  // exchange [k1][g][j]: k1 * XS + g * 17 + j
-  __shared__ float2 xb[16 * XS];  // exchange, then the tile [m][g] (m * 17 + g)
-  __shared__ float2 wb[256], t2s[256];
-  const RowD r = a.k.rows[a.rows[blockIdx.y]];
+  float2* const xb = lds;      // exchange, then the tile [m][g] (m * 17 + g)
+  float2* const wb = lds + 16 * XS;
+  float2* const t2s = wb + 256;
+  const RowD r = a.k.rows[a.rows[y]];
   const int N = a.k.N;
-  const int f0 = blockIdx.x << 4;
-  const int g = threadIdx.x & 15, j = threadIdx.x >> 4;
+  const int f0 = fblk << 4;
+  const int g = tid & 15, j = tid >> 4;
   {
-    const int k1 = threadIdx.x >> 4, jj = threadIdx.x & 15;
-    wb[threadIdx.x] = a.k.tw((unsigned)(k1 * jj) << LOG2F);
-    const int gg = threadIdx.x >> 4, k = threadIdx.x & 15;
-    t2s[threadIdx.x] = a.k.tw((unsigned)((16 * k * (f0 + gg)) & (N - 1)));
+    const int k1 = tid >> 4, jj = tid & 15;
+    wb[tid] = a.k.tw((unsigned)(k1 * jj) << LOG2F);
+    const int gg = tid >> 4, k = tid & 15;
+    t2s[tid] = a.k.tw((unsigned)((16 * k * (f0 + gg)) & (N - 1)));
   }
   const float2 base = cscale(a.k.tw((unsigned)(j * (f0 + g))), a.k.norm);
-  const gfloat2* U = (const gfloat2*)(a.scratch + (size_t)blockIdx.y * N);
+  const gfloat2* U = (const gfloat2*)(a.scratch + (size_t)y * N);
   float2 v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = gload(U + ((size_t)(j + 16 * k) << LOG2F) + f0 + g);
@@ -270,7 +288,7 @@ __global__ __launch_bounds__(256) void k_cols_out(BigArgs a) {
 #pragma unroll
   for (int k1 = 0; k1 < 16; ++k1) xb[k1 * XS + g * 17 + j] = v[k1];
   __syncthreads();
-  const int k1 = threadIdx.x >> 4;
+  const int k1 = tid >> 4;
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) v[jj] = xb[k1 * XS + g * 17 + jj];
   dft_reg<16, 1>(v);  // over j: output k2, m = k1 + 16 k2
@@ -281,10 +299,51 @@ __global__ __launch_bounds__(256) void k_cols_out(BigArgs a) {
   gfloat2* out = (gfloat2*)row_out(a.k, r);
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
-    const int e = s * 256 + threadIdx.x;
+    const int e = s * 256 + tid;
     const int m = e >> 4, gg = e & 15;
     const long long t = (long long)(f0 + gg) + ((long long)m << LOG2F);
     if (t < r.n) gstore_nt(out + t, xb[m * 17 + gg]);
+  }
+}
+
+// Pass 2 alone: grid (F / 16 f blocks, rows in the batch), 256 threads.
+template <int LOG2F>
+__global__ __launch_bounds__(256) void k_cols_out(BigArgs a) {
+  __shared__ float2 lds[ColsLds<LOG2F>::P2_ITEM];
+  cols_pass2<LOG2F>(a, blockIdx.y, blockIdx.x, threadIdx.x, lds);
+}
+
+// One launch of the column path: pass 1 of a batch of rows (p1, n1 rows)
+// and pass 2 of the previous batch (p2, n2 rows, in the other scratch set).
+// Their work-groups alternate while both kinds remain, so that the
+// arithmetic of pass 1 and the memory traffic of pass 2 share the CUs.
+// Grid: (256 / Qc) n1 + (F / 16) n2 / (T / 256) work-groups.
+template <int LOG2F>
+__global__ __launch_bounds__(ColsCfg<LOG2F>::T) void k_cols_step(BigArgs p1, int n1, BigArgs p2, int n2) {
+  using C = ColsCfg<LOG2F>;
+  constexpr int F = 1 << LOG2F;
+  constexpr int X1 = 256 >> C::LOG2QC;  // pass-1 work-groups per row
+  constexpr int SUB = C::T / 256;       // pass-2 items per work-group
+  constexpr int X2 = (F / 16) / SUB;    // pass-2 work-groups per row
+  static_assert((F / 16) % SUB == 0, "pass-2 items per row must fill whole work-groups");
+  __shared__ float2 lds[ColsLds<LOG2F>::SIZE];
+  const int w1 = X1 * n1, w2 = X2 * n2, lo = min(w1, w2);
+  const int w = blockIdx.x;
+  bool pass1;
+  int idx;
+  if (w < 2 * lo) {
+    pass1 = (w & 1) != 0;
+    idx = w >> 1;
+  } else {
+    pass1 = w1 > w2;
+    idx = lo + (w - 2 * lo);
+  }
+  if (pass1) {
+    cols_pass1<LOG2F>(p1, idx / X1, idx % X1, lds);
+  } else {
+    const int sub = threadIdx.x >> 8;
+    const int item = idx * SUB + sub;
+    cols_pass2<LOG2F>(p2, item / (F / 16), item % (F / 16), threadIdx.x & 255, lds + sub * ColsLds<LOG2F>::P2_ITEM);
   }
 }
 

@@ -273,7 +273,7 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
 //   base      = c W_N^{j (f0 + g)} for this thread, c the row normalisation
 template <int LOG2F>
 __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int f0,
-                                        const gfloat2* U, float2 base) {
+                                        const gfloat2* U, float2 base, unsigned* done) {
   float2* const buf = tm_buf;
   const float2* const t2 = tm_t2;
   const float2* const wb = tm_wb;
@@ -299,6 +299,9 @@ __device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int f0
   // consecutive f (128 bytes): 32-byte runs written straight from the
   // registers cost a fifth of the kernel's time
   __syncthreads();
+  // every thread's reads of U have landed (they fed the exchange above), so
+  // the buffer may be reused; the output stores need not complete
+  if (threadIdx.x == 0) add_agent(done, 1u);
 #pragma unroll
   for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = w[k2];
   __syncthreads();
@@ -466,11 +469,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
       if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error)) return;
-      team_pass2<LOG2F>((gfloat2*)row_out(ta.k, r), r.n, f0, U, base);
-      // every read of this buffer has landed in registers (they fed the LDS
-      // exchange), so it may be reused; the output stores need not complete
-      __syncthreads();
-      if (threadIdx.x == 0) add_agent(arrive + 64 * b + 32, 1u);
+      team_pass2<LOG2F>((gfloat2*)row_out(ta.k, r), r.n, f0, U, base, arrive + 64 * b + 32);
     }
   }
 }

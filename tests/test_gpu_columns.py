@@ -46,7 +46,7 @@ def test_column_path_against_oracle(n, fs, fn):
     got = run_device(x, sc, 4.0, plan=p)
     kinds = p.profile_read()
     p.profile(False)
-    assert {"row_columns", "row_writer_columns"} <= set(kinds) and "row_spreader" not in kinds, kinds
+    assert {"row_columns", "column_transpose"} <= set(kinds) and "row_spreader" not in kinds, kinds
     assert np.all(np.isfinite(got))
     rows = _rows_spanning_bands(n, sc, 4.0)
     ref = O.cwt_fp64(x, sc, 4.0, rows=rows)
