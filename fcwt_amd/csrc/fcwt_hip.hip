@@ -1066,6 +1066,8 @@ static void launch_cols_t(const BigArgs& b1, int n1, const BigArgs& b2, int n2, 
 
 static void launch_cols(int log2F, const BigArgs& b1, int n1, const BigArgs& b2, int n2, hipStream_t st) {
   switch (log2F) {
+    case 5: launch_cols_t<5>(b1, n1, b2, n2, st); break;
+    case 6: launch_cols_t<6>(b1, n1, b2, n2, st); break;
     case 7: launch_cols_t<7>(b1, n1, b2, n2, st); break;
     case 8: launch_cols_t<8>(b1, n1, b2, n2, st); break;
     case 9: launch_cols_t<9>(b1, n1, b2, n2, st); break;
@@ -1136,7 +1138,8 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     b.mnorm = p->m_norm;
     {
       ProfScope ps(p, 7, st);
-      k_spec_transpose<<<dim3(16 << (lf - 7), (unsigned)p->nsig), 256, 0, st>>>(p->d_spec, p->d_xt, lf);
+      k_spec_transpose<<<dim3(16 * std::max(1, (1 << lf) / 128), (unsigned)p->nsig), 256, 0, st>>>(p->d_spec, p->d_xt,
+                                                                                                lf);
     }
     const size_t set_elems = (size_t)p->cols_slots * p->N;
     const size_t nb = p->cols_batches.size();

@@ -1,5 +1,5 @@
 // rows_big.h -- band-limited inverse FFT for wide scales (band M > 1024 bins)
-// at padded sizes N = 256 F, 2^7 <= F <= 2^14, in two passes per batch of
+// at padded sizes N = 256 F, 2^5 <= F <= 2^14, in two passes per batch of
 // rows with the intermediate in one global scratch slot per row; each launch
 // runs pass 1 of one batch beside pass 2 of the previous one.
 // Device code, included by fcwt_hip.hip after rows_team.h.
@@ -26,7 +26,7 @@
 
 namespace fcwt_hip {
 
-constexpr int kColsMinLog2F = 7;
+constexpr int kColsMinLog2F = 5;
 constexpr int kColsMaxLog2F = 14;
 
 struct BigArgs {
@@ -38,7 +38,8 @@ struct BigArgs {
 };
 
 // Xt[s][qb][qf] = X_s[qb + 256 qf] for qb < 256, qf < F/2, through 64 x 16
-// LDS tiles; grid (16 * ceil(F/128), signals).
+// LDS tiles; grid (16 * ceil(F/128), signals).  (F/2 < 64 leaves part of the
+// one tile column unused.)
 __global__ __launch_bounds__(256) void k_spec_transpose(const float2* __restrict__ spec, float2* __restrict__ xt,
                                                         int log2F) {
   __shared__ float2 tile[64 * 17];
@@ -71,7 +72,15 @@ struct ColsCfg {
 // (stride 16) on distinct banks.
 __device__ __forceinline__ int cpad(int i) { return i + (i >> 4); }
 
-// W_F^r from the LDS tables t1[r & 127] = W_F^{r mod 128}, t2[r >> 7] = W_F^{128 (r >> 7)}
+// Twiddle tables of the column transforms: t1 holds W_F^r for r < min(F, 128),
+// t2 holds W_F^{128 s} for s < max(1, F/128).
+template <int LOG2F>
+struct ColsTw {
+  static constexpr int T1N = LOG2F < 7 ? (1 << LOG2F) : 128;
+  static constexpr int T2N = LOG2F < 7 ? 1 : (1 << LOG2F) / 128;
+};
+
+// W_F^r (r < F) from the LDS tables t1[r & 127] = W_F^{r mod 128}, t2[r >> 7] = W_F^{128 (r >> 7)}
 __device__ __forceinline__ float2 cols_tw(const float2* t1, const float2* t2, int r) {
   return cmul(t2[r >> 7], t1[r & 127]);
 }
@@ -153,13 +162,13 @@ __device__ __forceinline__ int cols_skip(int A, int* log2S) {
 }
 
 // LDS of the column kernel (complex64 values): pass 1 holds the columns
-// (PADW), its twiddle tables (128 + F/128) and the row's segments; pass 2
+// (PADW), its twiddle tables (128 + T2N) and the row's segments; pass 2
 // holds, per 256-thread item, the exchange/tile (16 x 272) and two factor
 // tables of 256.
 template <int LOG2F>
 struct ColsLds {
   using C = ColsCfg<LOG2F>;
-  static constexpr int P1 = C::PADW + 128 + (1 << LOG2F) / 128 + 2 * kTeamSegLds;
+  static constexpr int P1 = C::PADW + 128 + ColsTw<LOG2F>::T2N + 2 * kTeamSegLds;
   static constexpr int P2_ITEM = 16 * 272 + 512;
   static constexpr int P2 = (C::T / 256) * P2_ITEM;
   static constexpr int SIZE = P1 > P2 ? P1 : P2;
@@ -174,14 +183,14 @@ __device__ __forceinline__ void cols_pass1(const BigArgs& a, int y, int xblk, fl
   float2* const d = lds;
   float2* const t1 = lds + C::PADW;
   float2* const t2 = t1 + 128;
-  SegD* const sseg = reinterpret_cast<SegD*>(t2 + F / 128);
+  SegD* const sseg = reinterpret_cast<SegD*>(t2 + ColsTw<LOG2F>::T2N);
   const RowD r = a.k.rows[a.rows[y]];
   const int qb0 = xblk << C::LOG2QC;
-  for (int e = threadIdx.x; e < 128 + F / 128; e += C::T) {  // W_F^r = W_N^{256 r}
-    if (e < 128)
+  for (int e = threadIdx.x; e < ColsTw<LOG2F>::T1N + ColsTw<LOG2F>::T2N; e += C::T) {  // W_F^r = W_N^{256 r}
+    if (e < ColsTw<LOG2F>::T1N)
       t1[e] = a.k.tw((unsigned)e << 8);
     else
-      t2[e - 128] = a.k.tw((unsigned)(e - 128) << 15);
+      t2[e - ColsTw<LOG2F>::T1N] = a.k.tw((unsigned)(e - ColsTw<LOG2F>::T1N) << 15);
   }
   const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
   if ((int)threadIdx.x < nseg) sseg[threadIdx.x] = a.k.segs[r.seg_begin + threadIdx.x];

@@ -1,5 +1,5 @@
-"""GPU parity of the column path for wide bands (k_spec_transpose, k_cols,
-k_cols_out): padded sizes N = 2^15 .. 2^22 with bands up to N/2, against the
+"""GPU parity of the column path for wide bands (k_spec_transpose,
+k_cols_step, k_cols_out): padded sizes N = 2^13 .. 2^22 with bands up to N/2, against the
 float64 oracle, and against the generic two-pass kernels (FCWT_HIP_NO_COLS=1)
 on the same plan shape.
 
@@ -54,9 +54,10 @@ def test_column_path_against_oracle(n, fs, fn):
     assert err.max() <= TOL, f"row {rows[int(np.argmax(err))]}: rel-L2 {err.max():.3e}"
 
 
-@pytest.mark.parametrize("n", [20_000, 50_000, 100_000, 200_000])
+@pytest.mark.parametrize("n", [5000, 8192, 10_000, 16_384, 20_000, 50_000, 100_000, 200_000])
 def test_column_path_every_row(n, monkeypatch):
-    """N = 2^15 .. 2^18 with the team path off: every row against the oracle."""
+    """N = 2^13 .. 2^18 (the team path off where it would apply): every row
+    against the oracle."""
     monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
     x = tones_noise(n, seed=n + 1)
     sc = O.scales(O.LOGSCALES, 1000, 0.5, 499, 48)
@@ -96,7 +97,7 @@ def test_column_path_low_frequency_rows(monkeypatch):
     assert err[nz].max() <= TOL, err.max()
 
 
-@pytest.mark.parametrize("n", [150_000, 1_500_000])
+@pytest.mark.parametrize("n", [10_000, 150_000, 1_500_000])
 def test_column_path_matches_two_pass_kernels(n, monkeypatch):
     """The same plan through the generic two-pass kernels: same rows, the
     float32 results agree to float32 rounding."""
