@@ -335,25 +335,34 @@ def _pmc_per_dispatch(csv_path, counter, names):
 
 
 TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --kernel-trace --stats --output-format csv "
-             "-d $GRAFT_REPO_ROOT/gpurun_out/prof/trace -o run -- python3 $GRAFT_REPO_ROOT/bench.py --steps 5 "
-             "--warmup 2 --no-cpu")
+             "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py {a}")
+
+# kernel traces summarized under profiles/: trace directory, file tag, title, bench.py arguments, note
+TRACES = [
+    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 5 --warmup 2 --no-cpu",
+     "Calls include the 2 warm-up steps and 5 timed steps of bench.py."),
+    ("trace_long", "cwt_long_4194304_x_1024", "long signal, n = 2^22 x 1024 LINFREQS (sigma 4)",
+     "--workload long --steps 2 --warmup 1",
+     "Calls include the warm-up step, the 2 timed steps and the profiled step of bench.py."),
+    ("trace_batched", "cwt_batched_384_channels", "384 channels x 200 LOGSCALES (sigma 6)",
+     "--workload batched --steps 2 --warmup 1",
+     "Calls include the plain and batched runs of bench.py (warm-up and timed)."),
+]
 
 
-def summarize_trace(root):
-    """profiles/cwt_100k_x_3000_kernel_trace.md from rocprofv3 --kernel-trace
-    --stats output in root/trace/run_kernel_stats.csv."""
+def summarize_trace(root, d, tag, title, args, note):
+    """profiles/<tag>_kernel_trace.md from rocprofv3 --kernel-trace --stats
+    output in root/<d>/run_kernel_stats.csv."""
     import csv
 
-    path = os.path.join(root, "trace", "run_kernel_stats.csv")
+    path = os.path.join(root, d, "run_kernel_stats.csv")
     with open(path) as f:
         rows = list(csv.DictReader(f))
-    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                       f"cwt_{WORKLOAD_TAG}_kernel_trace.md")
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{tag}_kernel_trace.md")
     with open(out, "w") as f:
-        f.write("# Kernel times, cwt 100k x 3000 (rocprofv3 --kernel-trace --stats)\n\n")
-        f.write(f"Command (on the GPU box):\n\n    {TRACE_CMD}\n\n")
-        f.write("Summarized by `python bench.py --summarize-profiles gpurun_out/prof`.\n"
-                "Calls include the 2 warm-up steps and 5 timed steps of bench.py.\n\n")
+        f.write(f"# Kernel times, {title} (rocprofv3 --kernel-trace --stats)\n\n")
+        f.write(f"Command (on the GPU box):\n\n    {TRACE_CMD.format(d=d, a=args)}\n\n")
+        f.write(f"Summarized by `python bench.py --summarize-profiles gpurun_out/prof`.\n{note}\n\n")
         f.write("| kernel | calls | average (us) | min (us) | max (us) | share (%) |\n|---|---|---|---|---|---|\n")
         for r in rows:
             f.write(f"| `{r['Name'][:80]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
@@ -362,8 +371,9 @@ def summarize_trace(root):
 
 
 def summarize_profiles(root):
-    if os.path.exists(os.path.join(root, "trace", "run_kernel_stats.csv")):
-        summarize_trace(root)
+    for spec in TRACES:
+        if os.path.exists(os.path.join(root, spec[0], "run_kernel_stats.csv")):
+            summarize_trace(root, *spec)
     if os.path.exists(os.path.join(root, "pmc_fetch", "run_counter_collection.csv")):
         summarize_pmc(root)
 
