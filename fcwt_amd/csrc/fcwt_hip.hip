@@ -395,7 +395,25 @@ __global__ void k_adj_reduce(const float2* P, long long row0, int count, int fn,
   const float maxidx = (float)N - 1.0f;
   float2* a = A + (size_t)blockIdx.y * half;
   float2 acc = lo == s0 ? make_float2(0.f, 0.f) : a[q];
-  for (long long r = lo; r < hi; ++r) {
+  long long r = lo;
+  // four rows at a time, their loads issued together; the sum keeps row order
+  for (; r + 4 <= hi; r += 4) {
+    float w[4];
+    float2 g[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      w[k] = row_window(segs, rows[r + k - s0], q, maxidx, mother);
+      g[k] = P[(size_t)(r + k - row0) * half + q];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (w[k] != 0.f) {
+        acc.x += w[k] * g[k].x;
+        acc.y += w[k] * g[k].y;
+      }
+    }
+  }
+  for (; r < hi; ++r) {
     const float w = row_window(segs, rows[r - s0], q, maxidx, mother);
     if (w != 0.f) {
       const float2 g = P[(size_t)(r - row0) * half + q];
