@@ -361,7 +361,7 @@ def summarize_trace(root, d, tag, title, args, note):
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{tag}_kernel_trace.md")
     with open(out, "w") as f:
         f.write(f"# Kernel times, {title} (rocprofv3 --kernel-trace --stats)\n\n")
-        f.write(f"Command (on the GPU box):\n\n    {TRACE_CMD.format(d=d, a=args)}\n\n")
+        f.write(f"Command (on the MI355X):\n\n    {TRACE_CMD.format(d=d, a=args)}\n\n")
         f.write(f"Summarized by `python bench.py --summarize-profiles gpurun_out/prof`.\n{note}\n\n")
         f.write("| kernel | calls | average (us) | min (us) | max (us) | share (%) |\n|---|---|---|---|---|---|\n")
         for r in rows:
@@ -419,7 +419,7 @@ def summarize_pmc(root):
         json.dump(summary, f, indent=1)
     with open(base + ".md", "w") as f:
         f.write(f"# HBM traffic of k_rows_team, cwt 100k x 3000 (rocprofv3 --pmc)\n\n")
-        f.write("Commands (one counter per run, on the GPU box):\n\n")
+        f.write("Commands (one counter per run, on the MI355X):\n\n")
         for c in summary["commands"]:
             f.write(f"    {c}\n")
         f.write(f"\nSummarized by `{summary['summarized_by']}`.\n\n")
@@ -448,7 +448,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         sys.exit("bench.py measures one GPU: FCWT::cwt runs on a single MI355X here "
-                 "(multi-GPU sharding is described in DESIGN.md, not run); use --gpus 1")
+                 "(the path shards by rows or channels across GPUs, but no multi-GPU run is made); use --gpus 1")
     if args.summarize_profiles:
         summarize_profiles(args.summarize_profiles)
         return

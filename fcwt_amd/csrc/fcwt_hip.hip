@@ -1306,7 +1306,7 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
     p->scales.insert(p->scales.end(), sc[s], sc[s] + fns[s]);
   }
 
-  // host copy of the mother table, used only to judge which stale bins matter
+  // host copy of the mother table, used only to decide which stale bins matter
   std::vector<float> mother_host(p->N);
   morlet_mother(fb, p->N, mother_host.data());
   TRY(build_schedule(p, mother_host));
