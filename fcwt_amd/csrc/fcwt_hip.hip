@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -513,6 +514,7 @@ struct fcwt_hip_plan_s {
   // XCD-team path for wide bands (rows_team.h)
   bool team_capable = false;
   int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, seq_stride = 0;
+  long team_spin = kTeamSpin;  // FCWT_HIP_TEAM_SPIN lowers it to force a timed-out wait in tests
   std::vector<int> team_rows, team_seq, team_blocks;
   int* d_team_rows = nullptr;
   int* d_team_seq = nullptr;
@@ -659,9 +661,23 @@ static void free_plan(fcwt_hip_plan p) {
   delete p;
 }
 
+// FCWT_HIP_DEVICE_BUDGET_MB=m caps the device memory one plan may hold (its
+// tables, spectra, schedule and scratch; not the caller's buffers): an
+// allocation beyond it fails with FCWT_HIP_ENOMEM, as a failed hipMalloc does.
+static size_t device_budget() {
+  const char* e = std::getenv("FCWT_HIP_DEVICE_BUDGET_MB");
+  const long long v = e ? std::atoll(e) : 0;
+  return v > 0 ? (size_t)v << 20 : SIZE_MAX;
+}
+
 template <class T>
 static int dalloc(fcwt_hip_plan p, T** ptr, size_t count) {
   if (count == 0) count = 1;
+  if (p->bytes + count * sizeof(T) > device_budget()) {
+    *ptr = nullptr;
+    return fail(FCWT_HIP_ENOMEM, "device allocation of " + std::to_string(count * sizeof(T)) +
+                                     " bytes exceeds the plan's budget (FCWT_HIP_DEVICE_BUDGET_MB)");
+  }
   hipError_t e = hipMalloc((void**)ptr, count * sizeof(T));
   if (e != hipSuccess) {
     *ptr = nullptr;
@@ -1138,6 +1154,7 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     ta.team_size = p->team_size;
     ta.log2F = p->team_log2F;
     ta.nbuf = p->team_nbuf;
+    ta.spin = p->team_spin;
     ta.scratch = p->d_team_scratch;
     ta.sync = p->d_team_sync;
     ta.error = p->d_error;
@@ -1299,6 +1316,10 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   p->normalize = normalize != 0;
   p->flags = flags;
   p->team_capable = team_capable(dev, log2N);
+  if (const char* e = std::getenv("FCWT_HIP_TEAM_SPIN")) {
+    const long v = std::atol(e);
+    if (v >= 1) p->team_spin = std::min(v, kTeamSpin);
+  }
   for (int s = 0; s < nsig; ++s) {
     p->sig_row0.push_back((int)p->scales.size());
     p->sig_n.push_back(lengths[s]);

@@ -47,7 +47,7 @@ constexpr size_t kTeamScratchPerXcd = 3u << 20;  // scratch bytes per team (defa
 constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
 constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths
-constexpr long kTeamSpin = 1L << 22;           // polls (with s_sleep) before a wait is declared failed
+constexpr long kTeamSpin = 1L << 22;           // default polls (with s_sleep) before a wait is declared failed
 
 struct TeamArgs {
   KArgs k;
@@ -59,6 +59,7 @@ struct TeamArgs {
   int team_size;           // work-groups per team
   int log2F;               // F = N / 256 (the kernel's template parameter)
   int nbuf;                // scratch buffers per team (2..kTeamBuffers)
+  long spin;               // polls before a wait is declared failed (kTeamSpin unless a test lowers it)
   float2* scratch;         // [team][nbuf][N]
   unsigned* sync;          // zeroed before launch; see below
   unsigned* error;         // set to a nonzero code on a failed wait or team formation
@@ -90,11 +91,13 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_sched_barrier(0);  // and keeps the register DFTs on either side apart
 }
 
-// Thread 0 waits until *p >= target (bounded); the result is broadcast.
-__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, int* s_flag, unsigned* error) {
+// Thread 0 waits until *p >= target (at most `spin` polls); the result is
+// broadcast.
+__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, int* s_flag, unsigned* error,
+                                          long spin) {
   if (threadIdx.x == 0) {
     long s = 0;
-    while (ld_agent(p) < target && ld_agent(error) == 0 && s < kTeamSpin) {
+    while (ld_agent(p) < target && ld_agent(error) == 0 && s < spin) {
       __builtin_amdgcn_s_sleep(1);
       ++s;
     }
@@ -364,7 +367,8 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     add_agent(&sync[8], 1u);
   }
   __syncthreads();
-  if (!team_wait(&sync[8], gridDim.x, &s_flag, error)) return;
+  const long spin = ta.spin;
+  if (!team_wait(&sync[8], gridDim.x, &s_flag, error, spin)) return;
   if (threadIdx.x == 0) {
     bool ok = true;
     for (int x = 0; x < 8; ++x) ok &= ld_agent(&sync[x]) == (unsigned)team_size;
@@ -449,7 +453,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // the buffer's previous row has been read by every pass-2 item (the
       // wait ends in a barrier, which also publishes the staged band)
       if (gen > 0) {
-        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error)) return;
+        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error, spin)) return;
       } else {
         __syncthreads();
       }
@@ -468,7 +472,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // W_N^{j f} times the row normalisation
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
-      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error)) return;
+      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error, spin)) return;
       team_pass2<LOG2F>((gfloat2*)row_out(ta.k, r), r.n, f0, U, base, arrive + 64 * b + 32);
     }
   }

@@ -1,0 +1,56 @@
+"""Failure handling on the GPU, with every failure forced from the host so
+that every kernel still exits normally: a device-memory budget that plan
+creation exceeds (the allocation-failure path), and a spin limit of one poll
+on the wide-band kernel's own bounded waits (the timed-out-wait path).  After
+each failure the library must report it once, hold no half-built plan, and
+compute correctly again."""
+import numpy as np
+import pytest
+
+import fcwt_amd
+from fcwt_amd import _native as nat
+from oracle import oracle as O
+from tests.gpu_util import run_device, tones_noise
+
+pytestmark = pytest.mark.gpu
+
+
+def test_allocation_failure_is_reported_and_recovers(monkeypatch):
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 3000)
+    monkeypatch.setenv("FCWT_HIP_DEVICE_BUDGET_MB", "1")
+    with pytest.raises(nat.FcwtError) as e:
+        fcwt_amd.Plan(100_000, sc, 2.0)
+    assert e.value.code == nat.ENOMEM and "budget" in str(e.value)
+    # a batch (one plan per padded size) fails the same way
+    with pytest.raises(nat.FcwtError):
+        fcwt_amd.Batch([50_000, 60_000], sc[:40], 2.0)
+    monkeypatch.delenv("FCWT_HIP_DEVICE_BUDGET_MB")
+    x = tones_noise(100_000, seed=3)
+    rows = [0, 1000, 2999]
+    got = run_device(x, sc, 2.0)[rows]
+    assert O.rel_l2_rows(got, O.cwt_fp64(x, sc, 2.0, rows=rows)).max() <= 1e-5
+
+
+def test_timed_out_wait_is_reported_once_and_recovers(monkeypatch):
+    """A spin limit of one poll: the first work-group to reach the team
+    formation wait finds the others missing, records the error, and every
+    work-group exits; plan_check reports it, and a later check is clean."""
+    import torch
+
+    x = tones_noise(100_000, seed=4)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 300)
+    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "1")
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    assert p.row_split()["team"] > 0
+    dx = torch.from_numpy(x).cuda()
+    out = torch.empty((sc.size, x.size), dtype=torch.complex64, device="cuda")
+    p.execute_device(dx.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    with pytest.raises(nat.FcwtError) as e:
+        p.check()
+    assert "timed out" in str(e.value)
+    p.check()  # reported once; the error word is clear again
+    monkeypatch.delenv("FCWT_HIP_TEAM_SPIN")
+    got = run_device(x, sc, 2.0)
+    rows = [0, 150, 299]
+    assert O.rel_l2_rows(got[rows], O.cwt_fp64(x, sc, 2.0, rows=rows)).max() <= 1e-5

@@ -51,6 +51,31 @@ def test_shape():
     assert np.isfinite(out).all() and np.abs(out).max() > 0
 
 
+@pytest.mark.gpu
+def test_complex_input():
+    """tests/test_fcwt.py:67-89: ccwt of sin + i sin, n = 2,000,000.  Beyond
+    the reference's shape check: the input is (1 + i) x for a real x, and only
+    the positive half-spectrum enters a row, so the output is (1 + i) times
+    the real path's output for x."""
+    fs, n = 100, 2_000_000
+    x = np.arange(n)
+    real = np.sin(2 * np.pi * ((1 + (7 * x) / n) * (x / fs)), dtype="single")
+    input = real + 1j * real
+    assert input.dtype == np.complex64
+    f0, f1, fn = 1, 32, 30
+    morl = Morlet(2.0)
+    out = np.zeros((fn, len(x)), dtype="csingle")
+    scales = Scales(morl, FCWT_LOGSCALES, fs, f0, f1, fn)
+    f = FCWT(morl, 8, True, True)
+    f.ccwt(input, scales, out)
+    assert out.shape[0] == fn and out.shape[1] == n
+    ref = np.zeros((fn, len(x)), dtype="csingle")
+    f.cwt(real, scales, ref)
+    expect = (1 + 1j) * ref.astype(np.complex128)
+    err = np.linalg.norm(out - expect, axis=1) / np.linalg.norm(expect, axis=1)
+    assert err.max() <= 2e-5, err.max()
+
+
 def test_wavelet():
     """tests/test_fcwt.py:91-99 (host only)"""
     morl = Morlet(2.0)
