@@ -188,7 +188,7 @@ def workload_batched(args):
             "config": {"workload": f"384 channels, n in [20000, 200000] (sum {total}), fs=30000, pink noise + "
                                    f"spikes, Morlet sigma=6, 200 LOGSCALES 10-10000 Hz, "
                                    f"{8 * fn * total / 1e9:.1f} GB output", "mode": "one batch execute",
-                       "plans": batch.shape(), "plain_ms": round(ms_plain, 3),
+                       "plans": batch.shape(), "graph_replays": batch.graph_launches(), "plain_ms": round(ms_plain, 3),
                        "batched_vs_plain_speedup": round(ms_plain / ms, 3)}}
 
 

@@ -293,6 +293,12 @@ class Batch:
     def check(self) -> None:
         nat.check(nat.lib().fcwt_hip_batch_check(self._h), "fcwt_hip_batch_check")
 
+    def graph_launches(self) -> int:
+        """Executes served by replaying a captured HIP graph so far."""
+        n = ctypes.c_longlong()
+        nat.check(nat.lib().fcwt_hip_batch_graph_launches(self._h, ctypes.byref(n)), "fcwt_hip_batch_graph_launches")
+        return n.value
+
     def close(self):
         if getattr(self, "_h", None):
             nat.lib().fcwt_hip_batch_destroy(self._h)

@@ -1681,9 +1681,28 @@ struct fcwt_hip_batch_s {
   };
   std::vector<Group> groups;
   int count = 0;
-  std::vector<const float*> xs;  // per-group gathering buffers
-  std::vector<float2*> outs;
+  // Graph replay: once the same input and output pointers come twice in a
+  // row, that execute's launches are captured (on `cap`) into a graph, which
+  // later executes with those pointers replay on the caller's stream.  Any
+  // execute with other pointers drops the graph (the plans' device pointer
+  // arrays then change).  Capture failing, or FCWT_HIP_NO_GRAPHS=1, leaves
+  // the batch eager.
+  std::vector<const void*> last_key, graph_key;
+  int repeats = 0;
+  bool graphs = true;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long long graph_launches = 0;
 };
+
+static void drop_graph(fcwt_hip_batch b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key.clear();
+}
 
 int fcwt_hip_batch_create(fcwt_hip_batch* out, int count, const int* lengths, const float* const* scales,
                           const int* fns, float fb, int threads, int normalize, unsigned flags) {
@@ -1695,6 +1714,7 @@ int fcwt_hip_batch_create(fcwt_hip_batch* out, int count, const int* lengths, co
   fcwt_hip_batch b = new (std::nothrow) fcwt_hip_batch_s;
   if (!b) return fail(FCWT_HIP_ENOMEM, "fcwt_hip_batch_create: out of host memory");
   b->count = count;
+  if (const char* e = std::getenv("FCWT_HIP_NO_GRAPHS")) b->graphs = e[0] != '1';
   std::vector<int> order(count);
   for (int c = 0; c < count; ++c) order[c] = c;
   std::stable_sort(order.begin(), order.end(),
@@ -1727,6 +1747,8 @@ int fcwt_hip_batch_create(fcwt_hip_batch* out, int count, const int* lengths, co
 
 int fcwt_hip_batch_destroy(fcwt_hip_batch b) {
   if (!b) return FCWT_HIP_OK;
+  drop_graph(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
   for (auto& g : b->groups) free_plan(g.plan);
   delete b;
   return FCWT_HIP_OK;
@@ -1737,22 +1759,82 @@ int fcwt_hip_batch_execute_device(fcwt_hip_batch b, const float* const* d_inputs
   if (!b || !d_inputs || !d_outputs) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_execute_device: null argument");
   for (int c = 0; c < b->count; ++c)
     if (!d_inputs[c] || !d_outputs[c]) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_execute_device: null pointer");
+  hipStream_t st = (hipStream_t)stream;
   int cur = 0;
   HIP_TRY(hipGetDevice(&cur));
+  const int dev = b->groups[0].plan->device;
+  if (cur != dev) HIP_TRY(hipSetDevice(dev));
+  std::vector<const void*> key(d_inputs, d_inputs + b->count);
+  key.insert(key.end(), d_outputs, d_outputs + b->count);
+  bool profiling = false;
+  for (auto& g : b->groups) profiling |= g.plan->profiling;
   int rc = FCWT_HIP_OK;
-  for (auto& g : b->groups) {
-    if (cur != g.plan->device) HIP_TRY(hipSetDevice(g.plan->device));
-    b->xs.clear();
-    b->outs.clear();
-    for (int c : g.members) {
-      b->xs.push_back(d_inputs[c]);
-      b->outs.push_back((float2*)d_outputs[c]);
-    }
-    rc = launch_all(g.plan, b->xs.data(), false, b->outs.data(), (hipStream_t)stream);
-    if (rc != FCWT_HIP_OK) break;
+  if (b->exec && key == b->graph_key && !profiling) {
+    hipError_t e = hipGraphLaunch(b->exec, st);
+    if (e == hipSuccess) ++b->graph_launches;
+    rc = e == hipSuccess ? FCWT_HIP_OK : fail(FCWT_HIP_EDEVICE, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    if (cur != dev) (void)hipSetDevice(cur);
+    return rc;
   }
-  (void)hipSetDevice(cur);
+  if (key != b->graph_key) drop_graph(b);
+  if (key == b->last_key) {
+    ++b->repeats;
+  } else {
+    b->repeats = 0;
+    b->last_key = key;
+  }
+  // each group's signals, in plan order
+  std::vector<std::vector<const float*>> xs(b->groups.size());
+  std::vector<std::vector<float2*>> outs(b->groups.size());
+  for (size_t k = 0; k < b->groups.size(); ++k)
+    for (int c : b->groups[k].members) {
+      xs[k].push_back(d_inputs[c]);
+      outs[k].push_back((float2*)d_outputs[c]);
+    }
+  if (b->graphs && b->repeats >= 1 && !profiling) {
+    // pointer arrays first, on the caller's stream (not part of the graph)
+    for (size_t k = 0; k < b->groups.size() && rc == FCWT_HIP_OK; ++k)
+      if (b->groups[k].plan->nsig > 1) rc = upload_pointers(b->groups[k].plan, xs[k].data(), outs[k].data(), st);
+    if (rc != FCWT_HIP_OK) {
+      if (cur != dev) (void)hipSetDevice(cur);
+      return rc;
+    }
+    bool ok = b->cap || hipStreamCreate(&b->cap) == hipSuccess;
+    ok = ok && hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+      int lrc = FCWT_HIP_OK;
+      for (size_t k = 0; k < b->groups.size() && lrc == FCWT_HIP_OK; ++k)
+        lrc = launch_all(b->groups[k].plan, xs[k].data(), false, outs[k].data(), b->cap);
+      hipGraph_t g = nullptr;
+      ok = hipStreamEndCapture(b->cap, &g) == hipSuccess && lrc == FCWT_HIP_OK && g;
+      if (ok) {
+        b->graph = g;
+        ok = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) == hipSuccess;
+      } else if (g) {
+        (void)hipGraphDestroy(g);
+      }
+    }
+    if (ok && hipGraphLaunch(b->exec, st) == hipSuccess) {
+      b->graph_key = key;
+      ++b->graph_launches;
+      if (cur != dev) (void)hipSetDevice(cur);
+      return FCWT_HIP_OK;
+    }
+    // capture is not available here: stay eager from now on
+    drop_graph(b);
+    b->graphs = false;
+    (void)hipGetLastError();
+  }
+  for (size_t k = 0; k < b->groups.size() && rc == FCWT_HIP_OK; ++k)
+    rc = launch_all(b->groups[k].plan, xs[k].data(), false, outs[k].data(), st);
+  if (cur != dev) (void)hipSetDevice(cur);
   return rc;
+}
+
+int fcwt_hip_batch_graph_launches(fcwt_hip_batch b, long long* launches) {
+  if (!b || !launches) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_graph_launches: null argument");
+  *launches = b->graph_launches;
+  return FCWT_HIP_OK;
 }
 
 int fcwt_hip_batch_check(fcwt_hip_batch b) {

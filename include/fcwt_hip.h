@@ -227,9 +227,16 @@ int fcwt_hip_batch_create(fcwt_hip_batch* batch, int count, const int* lengths, 
 int fcwt_hip_batch_destroy(fcwt_hip_batch batch);
 
 /* d_inputs[c]: lengths[c] float32 samples, d_outputs[c]: fns[c]*lengths[c]
- * complex64 (host arrays of `count` device pointers), enqueued on `stream`. */
+ * complex64 (host arrays of `count` device pointers), enqueued on `stream`.
+ * When the same pointers come twice in a row, the launches are captured into
+ * a HIP graph, and later executes with those pointers replay it (one graph
+ * launch); other pointers drop it.  FCWT_HIP_NO_GRAPHS=1 keeps every execute
+ * eager. */
 int fcwt_hip_batch_execute_device(fcwt_hip_batch batch, const float* const* d_inputs, float* const* d_outputs,
                                   void* stream);
+
+/* Executes of the batch that were served by a graph launch so far. */
+int fcwt_hip_batch_graph_launches(fcwt_hip_batch batch, long long* launches);
 
 /* As fcwt_hip_plan_check, for every plan of the batch. */
 int fcwt_hip_batch_check(fcwt_hip_batch batch);

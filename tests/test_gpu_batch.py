@@ -72,3 +72,44 @@ def test_new_output_buffers_between_calls():
     for x, f, s in zip(xs, first, second):
         assert np.array_equal(f, run_device(x, sc, 2.0))
         assert np.array_equal(s, run_device(x[::-1].copy(), sc, 2.0))
+
+
+def test_graph_replays_match_eager_execution(monkeypatch):
+    """Executes with the same buffers replay a captured HIP graph.  Every
+    replay, with eager work and synchronisations between them, equals the
+    eager result bit for bit; other output buffers drop the graph and stay
+    correct, and coming back captures again."""
+    lengths = [30_000, 70_000, 150_000, 160_000]
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 48)
+    xs = [tones_noise(n, seed=n) for n in lengths]
+    monkeypatch.setenv("FCWT_HIP_NO_GRAPHS", "1")
+    eager = fcwt_amd.Batch(lengths, sc, 2.0)
+    ref = _run_batch(eager, xs, [48] * 4)
+    assert eager.graph_launches() == 0
+    monkeypatch.delenv("FCWT_HIP_NO_GRAPHS")
+    b = fcwt_amd.Batch(lengths, sc, 2.0)
+    dx = [torch.from_numpy(x).cuda() for x in xs]
+    out_a = [torch.empty((48, n), dtype=torch.complex64, device="cuda") for n in lengths]
+    out_b = [torch.empty((48, n), dtype=torch.complex64, device="cuda") for n in lengths]
+    st = torch.cuda.current_stream()
+
+    def run(outs):
+        for o in outs:
+            o.zero_()
+        b.execute_device([d.data_ptr() for d in dx], [o.data_ptr() for o in outs], st.cuda_stream)
+        torch.cuda.synchronize()
+        b.check()
+        return [o.cpu().numpy() for o in outs]
+
+    for k in range(5):
+        for g, r in zip(run(out_a), ref):
+            assert np.array_equal(g, r), k
+        # the program's own eager work between replays
+        assert torch.isfinite(torch.randn(1 << 20, device="cuda").cumsum(0)).all()
+    assert b.graph_launches() == 4  # the first execute is eager, the second captures
+    for g, r in zip(run(out_b), ref):
+        assert np.array_equal(g, r)
+    for k in range(3):
+        for g, r in zip(run(out_a), ref):
+            assert np.array_equal(g, r), k
+    assert b.graph_launches() == 6
