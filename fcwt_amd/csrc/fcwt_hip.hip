@@ -468,17 +468,24 @@ using namespace fcwt_hip;
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_last_error;
 
-// Tuned schedule parameters per padded size N (fcwt_hip_tune_load), and the
-// tuner's own override while it times candidates.
+// Tuned schedule parameters per padded size N (fcwt_hip_tune,
+// fcwt_hip_tune_load), and the tuner's own override while it times
+// candidates.  Zero fields keep the built-in choice.
+struct Tuned {
+  int team_buffers = 0;  // XCD-team scratch buffers per team
+  int cols_mb = 0;       // column path: scratch MB per batch of rows
+  int path = 0;          // wide rows: 0 built-in, 1 XCD-team kernel, 2 column path
+};
 static std::mutex g_tune_mu;
-static std::map<int, int> g_tuned_buffers;  // N -> team scratch buffers
-static thread_local int t_force_buffers = 0;
+static std::map<int, Tuned> g_tuned;
+static thread_local bool t_forcing = false;
+static thread_local Tuned t_force;
 
-static int tuned_buffers(int N) {
-  if (t_force_buffers) return t_force_buffers;
+static Tuned tuned(int N) {
+  if (t_forcing) return t_force;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  auto it = g_tuned_buffers.find(N);
-  return it == g_tuned_buffers.end() ? 0 : it->second;
+  auto it = g_tuned.find(N);
+  return it == g_tuned.end() ? Tuned{} : it->second;
 }
 
 static int fail(int code, const std::string& msg) {
@@ -717,7 +724,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   p->rows.resize(p->fn);
   std::vector<int> wide, team, cols;
   const int team_log2F = p->log2N - kTeamLog2B;
-  const bool team_ok = p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F;
+  const Tuned tn = tuned(p->N);
+  const bool team_ok =
+      p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F && tn.path != 2;
   const bool cols_ok = team_log2F >= kColsMinLog2F && team_log2F <= kColsMaxLog2F && !cols_disabled();
   for (int i = 0, sgi = 0; i < p->fn; ++i) {
     while (i >= p->sig_row0[sgi] + p->sig_fn[sgi]) ++sgi;
@@ -757,7 +766,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     // (fcwt_hip_tune_load); FCWT_HIP_TEAM_BUFFERS overrides both
     const size_t row_bytes = sizeof(float2) * (size_t)p->N;
     p->team_nbuf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
-    if (const int tb = tuned_buffers(p->N)) p->team_nbuf = tb;
+    if (tn.team_buffers) p->team_nbuf = tn.team_buffers;
     if (const char* e = std::getenv("FCWT_HIP_TEAM_BUFFERS")) {
       int v = std::atoi(e);
       if (v >= 2 && v <= kTeamBuffers) p->team_nbuf = v;
@@ -792,7 +801,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     // batch are meant to stay in the memory-side cache between the passes
     p->cols_log2F = team_log2F;
     const size_t per_row = sizeof(float2) * (size_t)p->N;
-    size_t budget = kColsScratchBytes;
+    size_t budget = tn.cols_mb ? (size_t)tn.cols_mb << 20 : kColsScratchBytes;
     if (const char* e = std::getenv("FCWT_HIP_COLS_SCRATCH_MB")) {
       long v = std::atol(e);
       if (v >= 1 && v <= 4096) budget = (size_t)v << 20;
@@ -1847,17 +1856,20 @@ int fcwt_hip_batch_check(fcwt_hip_batch b) {
 }
 
 // ------------------------------------------------------------- tuning
-// Times one plan of n = N samples (the first N - N/4) and 1000 LOGSCALES
-// 1-450 Hz at fs = 1000 with `buffers` team scratch buffers; ms per execute.
-static int tune_time(int N, int threads, int buffers, int reps, float* ms) {
-  const int n = std::max(1, N - N / 4), fn = 1000;
+// Times one plan of n = N - N/4 samples and fn LOGSCALES 1-450 Hz at
+// fs = 1000 (3000 rows, fewer where the output would pass 3 GB) with the
+// schedule parameters `cand`; ms per execute.
+static int tune_time(int N, int threads, const Tuned& cand, int reps, float* ms) {
+  const int n = std::max(1, N - N / 4);
+  const int fn = (int)std::max<long long>(16, std::min<long long>(3000, 3000000000LL / (8LL * n)));
   std::vector<float> sc(fn), x(n);
   scales(1, 1000, 1.0f, 450.0f, fn, sc.data());
   for (int t = 0; t < n; ++t) x[t] = std::sin(0.05f * t) + 0.5f * std::sin(0.9f * t + 1.0f);
-  t_force_buffers = buffers;
+  t_force = cand;
+  t_forcing = true;
   fcwt_hip_plan p = nullptr;
   int rc = fcwt_hip_plan_create(&p, n, sc.data(), fn, 2.0f, threads, 1, 0);
-  t_force_buffers = 0;
+  t_forcing = false;
   if (rc != FCWT_HIP_OK) return rc;
   float* dx = nullptr;
   float* dout = nullptr;
@@ -1900,29 +1912,42 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
     if (rc != FCWT_HIP_OK) return rc;
     teams = team_capable(dev, log2N) && log2F >= kTeamMinLog2F && log2F <= kTeamMaxLog2F;
   }
+  const bool cols = log2F >= kColsMinLog2F && log2F <= kColsMaxLog2F;
   const size_t row_bytes = sizeof(float2) * (size_t)N;
-  int best_buf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+  // the built-in choice, written as is for effort 0 or sizes with nothing to choose
+  Tuned best;
+  best.team_buffers = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+  best.cols_mb = (int)(kColsScratchBytes >> 20);
+  best.path = 0;  // the built-in routing unless a timed candidate wins
   float best_ms = -1.f;
-  if (teams && effort > 0) {
-    const int reps = effort == 1 ? 1 : 3;
-    for (int b : {2, 3, 4, 6, 8, 12}) {
-      if (b > kTeamBuffers) continue;
+  if (effort > 0 && (teams || cols)) {
+    // candidates: the team kernel's buffer counts, and the column path's
+    // scratch per batch (beside the team kernel where it runs too)
+    std::vector<Tuned> cand;
+    if (teams)
+      for (int b : {2, 3, 4, 6, 8, 12})
+        if (effort == 2 || b <= 6) cand.push_back({b, best.cols_mb, 1});
+    for (int mb : {32, 64, 128, 256, 512})
+      if (effort == 2 || (mb >= 64 && mb <= 256)) cand.push_back({best.team_buffers, mb, 2});
+    const int reps = effort == 1 ? 3 : 7;  // timed runs per candidate (the fastest counts)
+    for (const Tuned& c : cand) {
       float ms = 0.f;
-      rc = tune_time(N, threads, b, reps, &ms);
+      rc = tune_time(N, threads, c, reps, &ms);
       if (rc != FCWT_HIP_OK) return rc;
       if (best_ms < 0.f || ms < best_ms) {
         best_ms = ms;
-        best_buf = b;
+        best = c;
       }
     }
   }
   std::ofstream f(path);
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   f << "fcwt_hip tuning 1\narch gfx950\nN " << N << "\nthreads " << threads << "\neffort " << effort
-    << "\nteam_buffers " << best_buf << "\ntime_ms " << best_ms << "\n";
+    << "\nteam_buffers " << best.team_buffers << "\ncols_mb " << best.cols_mb << "\npath "
+    << (best.path == 1 ? "team" : best.path == 2 ? "cols" : "default") << "\ntime_ms " << best_ms << "\n";
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned_buffers[N] = best_buf;
+  g_tuned[N] = best;
   return FCWT_HIP_OK;
 }
 
@@ -1930,8 +1955,9 @@ int fcwt_hip_tune_load(const char* path) {
   if (!path) return fail(FCWT_HIP_EINVAL, "fcwt_hip_tune_load: null path");
   std::ifstream f(path);
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: cannot read ") + path);
-  std::string line, key, arch;
-  int N = 0, buffers = 0, version = 0;
+  std::string line, key, arch, pth;
+  int N = 0, version = 0;
+  Tuned t;
   if (!std::getline(f, line) || std::sscanf(line.c_str(), "fcwt_hip tuning %d", &version) != 1 || version != 1)
     return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: not a tuning file: ") + path);
   while (std::getline(f, line)) {
@@ -1939,13 +1965,17 @@ int fcwt_hip_tune_load(const char* path) {
     is >> key;
     if (key == "arch") is >> arch;
     else if (key == "N") is >> N;
-    else if (key == "team_buffers") is >> buffers;
+    else if (key == "team_buffers") is >> t.team_buffers;
+    else if (key == "cols_mb") is >> t.cols_mb;
+    else if (key == "path") is >> pth;
   }
+  t.path = pth == "team" ? 1 : pth == "cols" ? 2 : 0;
   if (arch != "gfx950") return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: tuned for another GPU: ") + path);
-  if (N < 2 || (N & (N - 1)) || buffers < 2 || buffers > kTeamBuffers)
+  if (N < 2 || (N & (N - 1)) || (t.team_buffers && (t.team_buffers < 2 || t.team_buffers > kTeamBuffers)) ||
+      t.cols_mb < 0 || t.cols_mb > 4096 || !(pth.empty() || pth == "team" || pth == "cols" || pth == "default"))
     return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: bad contents: ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned_buffers[N] = buffers;
+  g_tuned[N] = t;
   return FCWT_HIP_OK;
 }
 

@@ -43,7 +43,7 @@ constexpr int kTeamWgPerCu = 3;                // work-groups of the team on eve
 constexpr int kTeamWgPerXcd = 32 * kTeamWgPerCu;
 constexpr int kTeamPoints = 4096;              // points per item
 constexpr int kTeamBuffers = 12;               // most scratch buffers per team
-constexpr size_t kTeamScratchPerXcd = 3u << 20;  // scratch bytes per team (default buffer count)
+constexpr size_t kTeamScratchPerXcd = 6u << 20;  // scratch bytes per team (default buffer count)
 constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
 constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths

@@ -204,10 +204,12 @@ int fcwt_hip_adjoint_batch_device(fcwt_hip_plan plan, const float* d_grad, float
  * load_FFT_optimization_plan (src/fcwt/fcwt.cpp:313-393), which make and
  * read FFTW wisdom files n{N}_t{T}.wis.  fcwt_hip_tune times the schedule
  * choices of padded size N (power of two) on this GPU -- the XCD-team
- * kernel's scratch buffer count -- and writes the best to `path` (a short
- * text file) and to this process's table.  effort: 0 keeps the defaults
- * (FFTW_ESTIMATE), 1 times each candidate once (FFTW_MEASURE), 2 takes the
- * best of three (FFTW_PATIENT, FFTW_EXHAUSTIVE).  fcwt_hip_tune_load reads
+ * kernel's scratch buffer count, the column path's scratch per batch, and
+ * which of the two takes the wide rows where both can -- and writes the best
+ * to `path` (a short text file) and to this process's table.  effort: 0 keeps the defaults
+ * (FFTW_ESTIMATE), 1 takes the best of three runs of each candidate
+ * (FFTW_MEASURE), 2 the best of seven over a wider candidate set
+ * (FFTW_PATIENT, FFTW_EXHAUSTIVE).  fcwt_hip_tune_load reads
  * such a file; plans created afterwards for that N use it. */
 int fcwt_hip_tune(int N, int threads, int effort, const char* path);
 int fcwt_hip_tune_load(const char* path);

@@ -123,3 +123,31 @@ def test_column_path_linearity_full_rows():
     yb = run_device(b, sc, 4.0, plan=p)
     yab = run_device(a + b, sc, 4.0, plan=p)
     assert O.rel_l2_rows(yab, ya + yb).max() < 3e-5
+
+
+def test_tuning_file_routes_wide_rows(tmp_path):
+    """A tuning file that names the column path for N = 2^17 (what
+    fcwt_hip_tune writes when that path times faster there) moves the wide
+    rows off the XCD-team kernel; the result stays within the bar.  The
+    built-in routing is restored afterwards (the tuned table is process-wide)."""
+    from fcwt_amd import _native as nat
+
+    def load(path_choice):
+        f = tmp_path / f"n131072_t1_{path_choice}.fcwt_hip"
+        f.write_text("fcwt_hip tuning 1\narch gfx950\nN 131072\nthreads 1\neffort 1\nteam_buffers 3\n"
+                     f"cols_mb 128\npath {path_choice}\ntime_ms 1\n")
+        nat.check(nat.lib().fcwt_hip_tune_load(str(f).encode()), "fcwt_hip_tune_load")
+
+    x = tones_noise(100_000, seed=21)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 64)
+    load("cols")
+    try:
+        p = fcwt_amd.Plan(x.size, sc, 2.0)
+        split = p.row_split()
+        assert split["team"] == 0 and split["two_pass"] > 0, split
+        got = run_device(x, sc, 2.0, plan=p)
+    finally:
+        load("default")
+    assert fcwt_amd.Plan(x.size, sc, 2.0).row_split()["team"] > 0
+    ref = O.cwt_fp64(x, sc, 2.0)
+    assert O.rel_l2_rows(got, ref).max() <= TOL

@@ -160,8 +160,9 @@ def test_plan(tmp_path, monkeypatch, capsys):
     fc.create_FFT_optimization_plan(1 << 15, "FFTW_MEASURE")
     text = open("n32768_t8.fcwt_hip").read()
     assert "arch gfx950" in text and "team_buffers" in text
-    # cwt with optimisation schemes reads the file (no warning) and gives the
-    # same result as without
+    assert "cols_mb" in text and "path" in text
+    # cwt with optimisation schemes reads the file (no warning) and agrees
+    # with the untuned result (the tuned path may sum in another order)
     scs = Scales(morl, FCWT_LOGSCALES, fs, 1, 32, 300)
     a = np.zeros((300, 30_000), np.complex64)
     xx = np.resize(x, 30_000)
@@ -170,6 +171,7 @@ def test_plan(tmp_path, monkeypatch, capsys):
     assert "WARNING" not in capsys.readouterr().out
     b = np.zeros_like(a)
     FCWT(morl, 8, False, False).cwt(xx, scs, b)
-    assert np.array_equal(a, b)
+    err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
+    assert err.max() <= 2e-6, err.max()
     FCWT(morl, 3, True, False).cwt(xx, scs, b)  # no file for T = 3
     assert "WARNING: Optimization scheme 'n32768_t3.fcwt_hip' was not found" in capsys.readouterr().out
