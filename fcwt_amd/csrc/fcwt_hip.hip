@@ -263,7 +263,10 @@ __device__ __forceinline__ float2 real_post(float2 zk, float2 zc, float2 w) {
 
 // Whole forward transform of one signal per work-group (H <= 2^14) with the
 // radix-16 register passes of fft_rows.h; grid dimension y indexes signals.
-template <int LOG2H>
+// DERIVED selects derived radix-16 twiddles (fft_rows.h): used for the
+// adjoint's transforms, not for the signal's spectrum, whose absolute error
+// every row inherits.
+template <int LOG2H, bool DERIVED>
 __global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
   __shared__ RowFftLds<LOG2H> s;
   constexpr int H = 1 << LOG2H;
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
   row_tw_tables<LOG2H>(s);
   for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[j] = packed(x, nx, j);
   __syncthreads();
-  row_fft<LOG2H, -1>(s);
+  row_fft<LOG2H, -1, DERIVED>(s);
   if (a.complex_in) {
     for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[k];
     return;
@@ -284,25 +287,33 @@ __global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
   }
 }
 
-static void launch_rows_fft(const FwdArgs& f, unsigned batch, hipStream_t st) {
+template <bool DERIVED>
+static void launch_rows_fft_t(const FwdArgs& f, unsigned batch, hipStream_t st) {
   const dim3 g(1, batch);
   switch (f.log2H) {
-    case 0: k_rows_fft<0><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 1: k_rows_fft<1><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 2: k_rows_fft<2><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 3: k_rows_fft<3><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 4: k_rows_fft<4><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 5: k_rows_fft<5><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 6: k_rows_fft<6><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 7: k_rows_fft<7><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 8: k_rows_fft<8><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 9: k_rows_fft<9><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 10: k_rows_fft<10><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 11: k_rows_fft<11><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 12: k_rows_fft<12><<<g, kRowFftThreads, 0, st>>>(f); break;
-    case 13: k_rows_fft<13><<<g, kRowFftThreads, 0, st>>>(f); break;
-    default: k_rows_fft<14><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 0: k_rows_fft<0, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 1: k_rows_fft<1, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 2: k_rows_fft<2, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 3: k_rows_fft<3, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 4: k_rows_fft<4, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 5: k_rows_fft<5, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 6: k_rows_fft<6, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 7: k_rows_fft<7, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 8: k_rows_fft<8, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 9: k_rows_fft<9, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 10: k_rows_fft<10, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 11: k_rows_fft<11, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 12: k_rows_fft<12, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    case 13: k_rows_fft<13, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
+    default: k_rows_fft<14, DERIVED><<<g, kRowFftThreads, 0, st>>>(f); break;
   }
+}
+
+static void launch_rows_fft(const FwdArgs& f, unsigned batch, hipStream_t st, bool derived = false) {
+  if (derived)
+    launch_rows_fft_t<true>(f, batch, st);
+  else
+    launch_rows_fft_t<false>(f, batch, st);
 }
 
 // Four-step pass 1: H1-point FFTs over j1 of z[j2 + H2*j1] for G consecutive
@@ -933,14 +944,15 @@ static void launch_direct(int log2M, unsigned blocks, const KArgs& a, hipStream_
 }
 
 // Forward transforms of `batch` complex signals of f.n floats each (strides
-// in f), keeping f.keep outputs: one work-group per signal up to 2^14 points,
-// the two four-step kernels beyond.
+// in f), keeping f.keep outputs: one work-group per signal up to 2^14 points
+// (with derived twiddles: these are the adjoint's transforms), the two
+// four-step kernels beyond.
 static void launch_fwd_complex(FwdArgs f, int batch, hipStream_t st) {
   f.complex_in = 1;
   f.log2H1 = f.log2H / 2;
   f.log2H2 = f.log2H - f.log2H1;
   if (f.log2H <= 14) {
-    launch_rows_fft(f, (unsigned)batch, st);
+    launch_rows_fft(f, (unsigned)batch, st, true);
   } else {
     int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
     int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));

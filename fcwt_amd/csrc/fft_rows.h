@@ -55,7 +55,10 @@ __device__ __forceinline__ void row_tw_tables(RowFftLds<LOG2H>& s) {
 }
 
 // One radix-R Stockham pass over s.data; Ns = 2^LOG2NS points already combined.
-template <int LOG2H, int LOG2R, int LOG2NS, int SIGN>
+// DERIVED: the radix-16 twiddles W^k come from two table reads (W^1, W^4)
+// and at most three products each, instead of one table pair per k (less
+// LDS traffic, a few ulp more error).
+template <int LOG2H, int LOG2R, int LOG2NS, int SIGN, bool DERIVED>
 __device__ __forceinline__ void row_fft_pass(RowFftLds<LOG2H>& s) {
   constexpr int H = 1 << LOG2H, R = 1 << LOG2R, Ns = 1 << LOG2NS;
   constexpr int GROUPS = H / R;
@@ -69,8 +72,27 @@ __device__ __forceinline__ void row_fft_pass(RowFftLds<LOG2H>& s) {
 #pragma unroll
       for (int k = 0; k < R; ++k) v[u][k] = s.data[g + k * GROUPS];
       if (LOG2NS > 0) {
+        constexpr int SH = LOG2H - LOG2NS - LOG2R;
+        if constexpr (R == 16 && DERIVED) {
+          // W^k for k < 16 from W^1 and W^4 (tables), each at most three
+          // products deep
+          float2 w[16];
+          w[1] = row_tw<LOG2H, SIGN>(s, jj << SH);
+          w[4] = row_tw<LOG2H, SIGN>(s, (4 * jj) << SH);
+          w[2] = cmul(w[1], w[1]);
+          w[3] = cmul(w[2], w[1]);
+          w[8] = cmul(w[4], w[4]);
+          w[12] = cmul(w[8], w[4]);
 #pragma unroll
-        for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], row_tw<LOG2H, SIGN>(s, (jj * k) << (LOG2H - LOG2NS - LOG2R)));
+          for (int b = 4; b < 16; b += 4)
+#pragma unroll
+            for (int k = 1; k < 4; ++k) w[b + k] = cmul(w[b], w[k]);
+#pragma unroll
+          for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], w[k]);
+        } else {
+#pragma unroll
+          for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], row_tw<LOG2H, SIGN>(s, (jj * k) << SH));
+        }
       }
       dft_reg<R, SIGN>(v[u]);
     }
@@ -91,13 +113,13 @@ __device__ __forceinline__ void row_fft_pass(RowFftLds<LOG2H>& s) {
 
 // In-place 2^LOG2H-point DFT of s.data, out[k] = sum_j in[j] e^{SIGN 2 pi i jk/H}.
 // Needs a __syncthreads() after filling s.data and the tables.
-template <int LOG2H, int SIGN>
+template <int LOG2H, int SIGN, bool DERIVED = false>
 __device__ void row_fft(RowFftLds<LOG2H>& s) {
   constexpr int FIRST = LOG2H % 4;  // radix of the first pass (0: none)
-  if constexpr (FIRST > 0) row_fft_pass<LOG2H, FIRST, 0, SIGN>(s);
-  if constexpr (LOG2H >= FIRST + 4) row_fft_pass<LOG2H, 4, FIRST, SIGN>(s);
-  if constexpr (LOG2H >= FIRST + 8) row_fft_pass<LOG2H, 4, FIRST + 4, SIGN>(s);
-  if constexpr (LOG2H >= FIRST + 12) row_fft_pass<LOG2H, 4, FIRST + 8, SIGN>(s);
+  if constexpr (FIRST > 0) row_fft_pass<LOG2H, FIRST, 0, SIGN, DERIVED>(s);
+  if constexpr (LOG2H >= FIRST + 4) row_fft_pass<LOG2H, 4, FIRST, SIGN, DERIVED>(s);
+  if constexpr (LOG2H >= FIRST + 8) row_fft_pass<LOG2H, 4, FIRST + 4, SIGN, DERIVED>(s);
+  if constexpr (LOG2H >= FIRST + 12) row_fft_pass<LOG2H, 4, FIRST + 8, SIGN, DERIVED>(s);
   static_assert(LOG2H <= 14, "one work-group holds at most 2^14 points");
 }
 

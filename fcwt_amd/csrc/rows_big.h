@@ -102,8 +102,27 @@ __device__ __forceinline__ void cols_pass(float2* d, const float2* t1, const flo
     for (int k = 0; k < R; ++k) v[u][k] = d[cpad(c0 + gl + (k << LOG2GPC))];
     if constexpr (LOG2NS > 0) {
       const int jj = gl & (Ns - 1);
+      constexpr int SH = LOG2F - LOG2NS - LOG2R;
+      if constexpr (R == 16) {
+        // W^k for k < 16 from W^1 and W^4 (tables), each at most three
+        // products deep
+        float2 w[16];
+        w[1] = cols_tw(t1, t2, jj << SH);
+        w[4] = cols_tw(t1, t2, (4 * jj) << SH);
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+        w[8] = cmul(w[4], w[4]);
+        w[12] = cmul(w[8], w[4]);
 #pragma unroll
-      for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], cols_tw(t1, t2, (jj * k) << (LOG2F - LOG2NS - LOG2R)));
+        for (int b = 4; b < 16; b += 4)
+#pragma unroll
+          for (int k = 1; k < 4; ++k) w[b + k] = cmul(w[b], w[k]);
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], w[k]);
+      } else {
+#pragma unroll
+        for (int k = 1; k < R; ++k) v[u][k] = cmul(v[u][k], cols_tw(t1, t2, (jj * k) << SH));
+      }
     }
     dft_reg<R, 1>(v[u]);
   }
