@@ -130,7 +130,8 @@ int fcwt_hip_plan_profile(fcwt_hip_plan plan, int enable);
 
 /* Waits for the recorded events and reports, per kernel kind that ran
  * ("forward_fft", "row_writer_direct", "row_spreader",
- * "row_writer_two_pass", "row_team", "adjoint"), the summed milliseconds and launch
+ * "row_writer_two_pass", "row_team", "adjoint", "row_columns",
+ * "column_transpose"), the summed milliseconds and launch
  * count since the last fcwt_hip_plan_profile.  Writes at most `max` entries;
  * *count gets the number of kinds that ran. */
 int fcwt_hip_plan_profile_read(fcwt_hip_plan plan, int max, const char** names, double* ms, long long* launches,
@@ -138,8 +139,8 @@ int fcwt_hip_plan_profile_read(fcwt_hip_plan plan, int max, const char** names, 
 
 /* How the plan splits its fn rows over the row kernels: rows written by the
  * single-work-group kernel ("row_writer_direct"), by the XCD-team kernel
- * ("row_team") and by the two-pass fallback ("row_spreader" +
- * "row_writer_two_pass").  Any pointer may be NULL.  Benchmarks use it to
+ * ("row_team") and by two-pass kernels: the column path ("row_columns") or
+ * the generic pair ("row_spreader" + "row_writer_two_pass").  Any pointer may be NULL.  Benchmarks use it to
  * price each kernel's output bytes; no reference counterpart. */
 int fcwt_hip_plan_row_split(fcwt_hip_plan plan, int* direct_rows, int* team_rows, int* two_pass_rows);
 
