@@ -94,6 +94,15 @@ def lib() -> ctypes.CDLL:
                     raise ImportError(
                         f"{LIB_PATH} is not built; run `python -m fcwt_amd.build` (the GPU library is required, "
                         "there is no CPU fallback)")
+                # torch ships its own HIP runtime under the same soname
+                # (libamdhip64.so.7) as /opt/rocm's; whichever loads first
+                # serves the whole process, and torch cannot find a GPU when
+                # the other one was loaded before it.  Loading torch first
+                # keeps one runtime, torch's, for both.
+                try:
+                    import torch  # noqa: F401
+                except ImportError:
+                    pass
                 L = ctypes.CDLL(str(LIB_PATH))
                 for name, (res, args) in SIGNATURES.items():
                     fn = getattr(L, name)

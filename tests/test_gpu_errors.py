@@ -54,3 +54,31 @@ def test_timed_out_wait_is_reported_once_and_recovers(monkeypatch):
     got = run_device(x, sc, 2.0)
     rows = [0, 150, 299]
     assert O.rel_l2_rows(got[rows], O.cwt_fp64(x, sc, 2.0, rows=rows)).max() <= 1e-5
+
+
+def test_library_before_torch_shares_one_runtime():
+    """A fresh process that calls the library before it touches torch: torch
+    bundles a HIP runtime under the same soname as /opt/rocm's, and the one
+    loaded first serves both, so loading the library must not leave torch
+    without a GPU.  The child runs fcwt.cwt on host arrays, then a torch
+    kernel, and checks the transform against the oracle."""
+    import os
+    import subprocess
+    import sys
+
+    code = (
+        "import numpy as np, fcwt_amd\n"
+        "from oracle import oracle as O\n"
+        "from tests.gpu_util import tones_noise\n"
+        "x = tones_noise(4000, fs=100, fmax=45, seed=3)\n"
+        "freqs, y = fcwt_amd.cwt(x, 100, 1, 40, 12)\n"
+        "import torch\n"
+        "t = torch.arange(8, dtype=torch.float32, device='cuda:0')\n"
+        "assert float(t.sum().item()) == 28.0\n"
+        "sc = O.scales(O.LINFREQS, 100, 1, 40, 12)\n"
+        "err = O.rel_l2_rows(y, O.cwt_fp64(x, sc, 2.0)).max()\n"
+        "assert err <= 1e-5, err\n"
+        "print('ok', err)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
