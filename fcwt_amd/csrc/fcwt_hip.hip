@@ -534,8 +534,8 @@ struct fcwt_hip_plan_s {
   int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, seq_stride = 0;
   long team_spin = kTeamSpin;  // FCWT_HIP_TEAM_SPIN lowers it to force a timed-out wait in tests
   std::vector<int> team_rows, team_seq, team_blocks;
-  int* d_team_rows = nullptr;
-  int* d_team_seq = nullptr;
+  std::vector<TeamBlk> team_desc;  // team_seq with each block's row fields
+  TeamBlk* d_team_desc = nullptr;
   int* d_team_blocks = nullptr;
   float2* d_team_scratch = nullptr;
   unsigned* d_team_sync = nullptr;
@@ -669,7 +669,7 @@ static void free_plan(fcwt_hip_plan p) {
   }
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
-                  p->d_team_rows, p->d_team_seq, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
+                  p->d_team_desc, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
                   p->d_adj_P, p->d_adj_atmp, p->d_adj_A, p->d_adj_y, p->d_grad, p->d_xbar,
                   p->d_sig_n, p->d_xs, p->d_outs, p->d_cols_rows, p->d_cols_scratch, p->d_xt};
   for (void* q : ptrs)
@@ -806,6 +806,20 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       }
       p->team_blocks[t] = len;
     }
+    p->team_desc.assign(p->team_seq.size(), TeamBlk{});
+    for (int t = 0; t < nt; ++t)
+      for (int s = 0; s < p->team_blocks[t]; ++s) {
+        TeamBlk& d = p->team_desc[(size_t)t * p->seq_stride + s];
+        d.sq = p->team_seq[(size_t)t * p->seq_stride + s];
+        const RowD& r = p->rows[p->team_rows[t + (size_t)nt * (d.sq >> 1)]];
+        d.seg_begin = r.seg_begin;
+        d.seg_count = r.seg_count;
+        d.log2A = r.log2A;
+        d.sig = r.sig;
+        d.n = r.n;
+        d.out_off = r.out_off;
+        for (int k = 0; k < 2 && k < r.seg_count; ++k) d.seg[k] = p->segs[(size_t)r.seg_begin + k];
+      }
   }
   if (!cols.empty()) {
     // batches of rows, one N-point scratch slot per row; the slots of one
@@ -1167,8 +1181,7 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     k_fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(p->d_team_sync, words, 0u);
     TeamArgs ta;
     ta.k = a;
-    ta.team_rows = p->d_team_rows;
-    ta.team_seq = p->d_team_seq;
+    ta.team_desc = p->d_team_desc;
     ta.team_blocks = p->d_team_blocks;
     ta.seq_stride = p->seq_stride;
     ta.nteams = p->nteams;
@@ -1385,11 +1398,10 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   }
   TRY(dalloc(p, &p->d_error, 1));
   if (p->nteams) {
-    TRY(dalloc(p, &p->d_team_rows, p->team_rows.size()));
-    TRY(dalloc(p, &p->d_team_seq, p->team_seq.size()));
+    TRY(dalloc(p, &p->d_team_desc, p->team_desc.size()));
     TRY(dalloc(p, &p->d_team_blocks, p->team_blocks.size()));
     TRY(dalloc(p, &p->d_team_scratch, (size_t)p->nteams * p->team_nbuf * N));
-    p->team_sync_words = (size_t)64 * (1 + p->nteams * kTeamBuffers);
+    p->team_sync_words = (size_t)64 * (1 + p->nteams * kTeamBuffers + p->nteams);
     TRY(dalloc(p, &p->d_team_sync, p->team_sync_words));
   }
 
@@ -1404,8 +1416,7 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   if (nsig > 1) TRY(up(p->d_sig_n, p->sig_n.data(), sizeof(int) * nsig));
   if (p->cols_slots) TRY(up(p->d_cols_rows, p->cols_rows.data(), p->cols_rows.size() * sizeof(int)));
   if (p->nteams) {
-    TRY(up(p->d_team_rows, p->team_rows.data(), p->team_rows.size() * sizeof(int)));
-    TRY(up(p->d_team_seq, p->team_seq.data(), p->team_seq.size() * sizeof(int)));
+    TRY(up(p->d_team_desc, p->team_desc.data(), p->team_desc.size() * sizeof(TeamBlk)));
     TRY(up(p->d_team_blocks, p->team_blocks.data(), p->team_blocks.size() * sizeof(int)));
   }
   k_fill_u32<<<1, 64, 0, p->stream>>>(p->d_error, 1, 0u);

@@ -40,30 +40,6 @@ struct DirectCfg {
   static constexpr int LDS = BUF + M /*Z*/ + M /*W_M^{j k1}*/ + R1 * G /*pre-twiddles*/;
 };
 
-// Z[q] for one row: X[q] * mother[(int)fminf(N-1, step*q)] in the segment
-// holding q (fcwt.cpp:224-248), 0 outside every segment.
-struct RowSpectrum {
-  const float2* spec;
-  const float* mother;
-  const SegD* segs;
-  int seg_begin, seg_count;
-  float maxidx;
-  __device__ __forceinline__ float2 operator()(int q) const {
-#pragma clang fp contract(off)
-    for (int s = 0; s < seg_count; ++s) {
-      SegD sg = segs[seg_begin + s];
-      if (q >= sg.lo && q < sg.hi) {
-        float v = sg.step * (float)q;
-        int idx = (int)(v < maxidx ? v : maxidx);
-        float w = mother[idx];
-        float2 x = spec[q];
-        return make_float2(x.x * w, x.y * w);
-      }
-    }
-    return make_float2(0.f, 0.f);
-  }
-};
-
 template <int LOG2M>
 __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   using C = DirectCfg<LOG2M>;
@@ -81,12 +57,44 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
   const int l0 = it.block * G;
   const int tid = threadIdx.x;
   {
-    RowSpectrum Z{row_spec(a, r), a.mother, a.segs, r.seg_begin, r.seg_count, (float)a.N - 1.0f};
-    for (int q = tid; q < M; q += NT) {
-      zs[q] = Z(q);
-      if (P2 > 1) {
-        const int j = q % P2, k1 = q / P2;  // j k1 < M
-        wm[q] = a.tw((unsigned)(j * k1) << log2L);
+    // Z[q] for this thread's bins: the segment scan runs once for all of
+    // them, then every gather is issued before the first one is used
+    constexpr int kZ = (M + NT - 1) / NT;
+    const float maxidx = (float)a.N - 1.0f;
+    int idx[kZ];
+#pragma unroll
+    for (int u = 0; u < kZ; ++u) idx[u] = -1;
+    for (int s = 0; s < r.seg_count; ++s) {
+#pragma clang fp contract(off)
+      const SegD sg = a.segs[r.seg_begin + s];
+#pragma unroll
+      for (int u = 0; u < kZ; ++u) {
+        const int q = tid + u * NT;
+        if (idx[u] < 0 && q >= sg.lo && q < sg.hi) {  // the first segment holding q
+          const float v = sg.step * (float)q;
+          idx[u] = (int)(v < maxidx ? v : maxidx);
+        }
+      }
+    }
+    const float2* const rspec = row_spec(a, r);
+    float2 xs[kZ];
+    float ws[kZ];
+#pragma unroll
+    for (int u = 0; u < kZ; ++u) {
+      const int q = tid + u * NT;
+      const bool in = q < M && idx[u] >= 0;
+      xs[u] = in ? rspec[q] : make_float2(0.f, 0.f);
+      ws[u] = in ? a.mother[idx[u]] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kZ; ++u) {
+      const int q = tid + u * NT;
+      if (q < M) {
+        zs[q] = make_float2(xs[u].x * ws[u], xs[u].y * ws[u]);
+        if (P2 > 1) {
+          const int j = q % P2, k1 = q / P2;  // j k1 < M
+          wm[q] = a.tw((unsigned)(j * k1) << log2L);
+        }
       }
     }
     // W_{R1 L}^{k l} = W_N^{k l P2}

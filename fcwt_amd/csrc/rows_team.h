@@ -15,9 +15,12 @@
 // One team per XCD: all work-groups the hardware places on that XCD (found
 // from the XCC id), so the scratch stays in the XCD's L2.  The team walks a
 // fixed sequence of item blocks -- step s holds pass 1 of row s and pass 2 of
-// row s - lag -- and member m takes items m, m + team_size, ...; every item
-// depends only on earlier ones, so the walk cannot deadlock while all members
-// are resident (cooperative launch).  A pass-1 item stores normally, waits
+// row s - lag.  Members claim items in sequence order from the team's
+// counter (an item claims its member's next one when it starts, so that the
+// claim's latency is hidden), which balances uneven items across members.
+// Every item depends only on earlier ones, and the smallest unfinished item
+// is always some member's current item, so the walk cannot deadlock while
+// all members are resident (cooperative launch).  A pass-1 item stores normally, waits
 // for its stores (s_waitcnt), then bumps its buffer's arrival counter; a
 // pass-2 item waits for the row's count and reads with device-scope loads,
 // which bypass the CU's L1 and are served by the shared L2, then bumps the
@@ -49,10 +52,21 @@ constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
 constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths
 constexpr long kTeamSpin = 1L << 22;           // default polls (with s_sleep) before a wait is declared failed
 
+// One block of a team's item sequence with what its items read about their
+// row, so that an item starts after one burst of scalar loads.
+struct TeamBlk {
+  int sq;                    // 2 i + pass: row i of the team, pass 0 or 1
+  int seg_begin, seg_count;  // the row's band segments in KArgs::segs
+  int log2A;                 // band M = 256 A
+  int sig, n;                // signal of the row and its length
+  long long out_off;         // offset of the row in its signal's output
+  SegD seg[2];               // the first two segments (any further ones stay in segs)
+};
+static_assert(sizeof(TeamBlk) == 64, "TeamBlk is read as 16 dwords");
+
 struct TeamArgs {
   KArgs k;
-  const int* team_rows;    // rows of team t: team_rows[t + nteams * i]
-  const int* team_seq;     // item blocks of team t: team_seq[t * seq_stride + s] = 2 i + pass (0 or 1)
+  const TeamBlk* team_desc;  // item blocks of team t: team_desc[t * seq_stride + s]
   const int* team_blocks;  // blocks in team t's sequence
   int seq_stride;
   int nteams;              // teams on the whole device (one per XCD)
@@ -66,7 +80,15 @@ struct TeamArgs {
 };
 // sync layout (unsigned, 32 per 128-byte line):
 //   [0..7] work-groups per XCD, [8] grid arrival counter,
-//   team t: arrive[b] at 64 * (1 + t * kTeamBuffers + b), done[b] 32 later.
+//   team t: arrive[b] at 64 * (1 + t * kTeamBuffers + b), done[b] 32 later;
+//   team t's next-item counter at 64 * (1 + nteams * kTeamBuffers + t).
+
+// Load through the constant address space. The schedule arrays are written
+// before the launch and never during it, so a uniform address becomes a
+// scalar load instead of a vector load and a readfirstlane.
+__device__ __forceinline__ int ld_const(const int* p) {
+  return *(const __attribute__((address_space(4))) int*)(p);
+}
 
 __device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -122,18 +144,26 @@ __device__ __forceinline__ int window_index(int q, int nseg, const SegD* sseg, f
   return (int)(v < maxidx ? v : maxidx);
 }
 
-// The same with the row's segments read from global memory: the address is
-// uniform over the work-group, so these are scalar loads, and no LDS copy
-// (with its barrier) is needed.
-__device__ __forceinline__ int window_index_g(int q, int nseg, const SegD* __restrict__ gseg, float maxidx) {
+// The same with the row's first two segments in registers (from TeamBlk)
+// and any further ones read by scalar loads.
+__device__ __forceinline__ int window_index_d(int q, int nseg, const SegD& s0, const SegD& s1,
+                                              const SegD* __restrict__ gseg, float maxidx) {
 #pragma clang fp contract(off)
   bool hit = false;
   float step = 0.f;
-  for (int s = 0; s < nseg; ++s) {
-    const SegD sg = gseg[s];
-    if (q >= sg.lo && q < sg.hi) {
+  if (nseg > 0 && q >= s0.lo && q < s0.hi) {
+    hit = true;
+    step = s0.step;
+  }
+  if (nseg > 1 && q >= s1.lo && q < s1.hi) {
+    hit = true;
+    step = s1.step;
+  }
+  for (int s = 2; s < nseg; ++s) {  // a later segment overrides an earlier one
+    const int* w = (const int*)(gseg + s);
+    if (q >= ld_const(w) && q < ld_const(w + 1)) {
       hit = true;
-      step = sg.step;
+      step = __int_as_float(ld_const(w + 2));
     }
   }
   if (!hit) return -1;
@@ -345,17 +375,15 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   float2* const post = tm_post;
   float2* const t2s = tm_t2;
   float2* const wb = tm_wb;
-  __shared__ int s_rank, s_flag;
+  __shared__ int s_rank, s_flag, s_next;
   // plain copies of the arguments (never take the address of the kernel argument)
   const float2* __restrict__ spec = ta.k.spec;
   const float* __restrict__ mother = ta.k.mother;
   const SegD* __restrict__ segs = ta.k.segs;
-  const RowD* __restrict__ rows = ta.k.rows;
   const Twiddles tw = ta.k.tw;
   const int N = ta.k.N;
   const float norm = ta.k.norm;
-  const int* __restrict__ team_rows = ta.team_rows;
-  const int nteams = ta.nteams, team_size = ta.team_size;
+  const int team_size = ta.team_size;
   constexpr int log2F = LOG2F;
   unsigned* const sync = ta.sync;
   unsigned* const error = ta.error;
@@ -378,7 +406,6 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   __syncthreads();
   if (!s_flag) return;
   const int team = xcc;
-  const int member = s_rank;
   constexpr int F = 1 << log2F;
   constexpr int log2Qb = 12 - log2F;
   constexpr int log2I = log2F - 4;  // items per pass per row: F/16
@@ -389,21 +416,47 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   const int nbuf = ta.nbuf;
   float2* const scratch = ta.scratch + (size_t)team * nbuf * N;
   unsigned* const arrive = sync + 64 * (1 + team * kTeamBuffers);
-  const int* __restrict__ seq = ta.team_seq + (size_t)team * ta.seq_stride;
+  const TeamBlk* const desc = ta.team_desc + (size_t)team * ta.seq_stride;
   const int items = ta.team_blocks[team] << log2I;
   int tab_log2A = -1;  // pass-1 tables currently built for this A
-  for (int it = member; it < items; it += team_size) {
-    const int sq = seq[it >> log2I], blk = it & ((1 << log2I) - 1);
+  // items are claimed in sequence order from the team's counter; an item
+  // claims its successor when it starts, so the claim's latency is hidden
+  unsigned* const next_item = sync + 64 * (1 + ta.nteams * kTeamBuffers + team);
+  if (threadIdx.x == 0) s_next = (int)atomicAdd(next_item, 1u);
+  __syncthreads();
+  int it = __builtin_amdgcn_readfirstlane(s_next);
+  while (it < items) {
+    unsigned claim = 0;
+    if (threadIdx.x == 0) claim = atomicAdd(next_item, 1u);
+    const int* const dw = (const int*)(desc + (it >> log2I));
+    const int sq = ld_const(dw), blk = it & ((1 << log2I) - 1);
     const int i = sq >> 1;
-    const RowD r = rows[team_rows[team + nteams * i]];
+    // every field is read here, in one burst, before any wait
+    const int seg_begin = ld_const(dw + 1);
+    const int nseg = ld_const(dw + 2);  // <= kTeamSegLds: the schedule routes rows with more elsewhere
+    const int log2A = ld_const(dw + 3);
+    const int sig = ld_const(dw + 4), n = ld_const(dw + 5);
+    const long long out_off = (long long)(unsigned)ld_const(dw + 6) | ((long long)ld_const(dw + 7) << 32);
+    SegD s0, s1;
+    s0.lo = ld_const(dw + 8);
+    s0.hi = ld_const(dw + 9);
+    s0.step = __int_as_float(ld_const(dw + 10));
+    s1.lo = ld_const(dw + 12);
+    s1.hi = ld_const(dw + 13);
+    s1.step = __int_as_float(ld_const(dw + 14));
+    float2* out = ta.k.out;  // the row's output (pass-2 items)
+    if (ta.k.outs) {
+      const int* const pw = (const int*)(ta.k.outs + sig);
+      out = (float2*)((unsigned long long)(unsigned)ld_const(pw) | ((unsigned long long)(unsigned)ld_const(pw + 1) << 32));
+    }
+    out += out_off;
     const int b = i % nbuf, gen = i / nbuf;
     gfloat2* const U = (gfloat2*)(scratch + (size_t)b * N);
     __syncthreads();  // the previous item's users of LDS are done
     if ((sq & 1) == 0) {
       // ---- pass 1 of row i, qb block blk
       const int qb0 = blk << log2Qb;
-      const int nseg = r.seg_count;  // <= kTeamSegLds: the schedule routes rows with more elsewhere
-      const int log2A = r.log2A, A = 1 << log2A;
+      const int A = 1 << log2A;
       if (log2A != tab_log2A) {
         tab_log2A = log2A;
         const int log2R1 = team_log2r1(log2A), R1 = 1 << log2R1;
@@ -424,7 +477,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // A * Qb = 2048 values, 8 per thread, all loads issued before the
       // first is used
       const float maxidx = (float)N - 1.0f;
-      const float2* __restrict__ rspec = spec + (size_t)r.sig * (size_t)(N >> 1);
+      const float2* __restrict__ rspec = spec + (size_t)sig * (size_t)(N >> 1);
       const int zstride = team_zs_stride(log2A);
       float2* const zs = buf + team_zs_offset(log2A, log2F);
       const int total = A << log2Qb;
@@ -435,7 +488,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
         const int e = u * kTeamThreads + threadIdx.x;
         const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
         qs[u] = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
-        idx[u] = e < total ? window_index_g(qs[u], nseg, segs + r.seg_begin, maxidx) : -1;
+        idx[u] = e < total ? window_index_d(qs[u], nseg, s0, s1, segs + seg_begin, maxidx) : -1;
       }
       float2 xs[kPer];
       float ws[kPer];
@@ -473,8 +526,11 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
       if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error, spin)) return;
-      team_pass2<LOG2F>((gfloat2*)row_out(ta.k, r), r.n, f0, U, base, arrive + 64 * b + 32);
+      team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
     }
+    if (threadIdx.x == 0) s_next = (int)claim;
+    __syncthreads();
+    it = __builtin_amdgcn_readfirstlane(s_next);
   }
 }
 
