@@ -452,7 +452,6 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     out += out_off;
     const int b = i % nbuf, gen = i / nbuf;
     gfloat2* const U = (gfloat2*)(scratch + (size_t)b * N);
-    __syncthreads();  // the previous item's users of LDS are done
     if ((sq & 1) == 0) {
       // ---- pass 1 of row i, qb block blk
       const int qb0 = blk << log2Qb;
@@ -528,6 +527,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error, spin)) return;
       team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
     }
+    // publishes the next item, and every thread is done with this item's LDS
     if (threadIdx.x == 0) s_next = (int)claim;
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(s_next);
