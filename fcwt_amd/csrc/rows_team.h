@@ -90,6 +90,12 @@ __device__ __forceinline__ int ld_const(const int* p) {
   return *(const __attribute__((address_space(4))) int*)(p);
 }
 
+// Waits until this thread's global stores are complete, i.e. written to the
+// L2 of its XCD, which the other members of the team read through.  (A
+// workgroup-scope release fence does not wait for global stores here: the
+// work-group's waves share one CU.)
+__device__ __forceinline__ void wait_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ unsigned ld_agent(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -301,11 +307,13 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
 
 // Pass 2 for one row: 16 consecutive f starting at f0, the factor W_N^{qb f}
 // and 256-point inverse FFTs over qb = j + 16 k, written to out (the row)
-// with non-temporal stores (written once, never re-read here).
+// with non-temporal stores (written once, never re-read here).  Inlined into
+// the kernel: a function's return waits for its stores, and these need not
+// complete before the member starts its next item.
 //   tm_t2[g][k]  = W_N^{16 k (f0 + g)}   tm_wb[k1][j] = W_B^{j k1}
 //   base      = c W_N^{j (f0 + g)} for this thread, c the row normalisation
 template <int LOG2F>
-__device__ __noinline__ void team_pass2(gfloat2* __restrict__ out, int n, int f0,
+__device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int f0,
                                         const gfloat2* U, float2 base, unsigned* done) {
   float2* const buf = tm_buf;
   const float2* const t2 = tm_t2;
@@ -510,8 +518,8 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
         __syncthreads();
       }
       team_pass1_for<LOG2F>(log2A, qb0, U);
-      // the pass-1 stores are complete (s_waitcnt) before the arrival is counted
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      // every thread's pass-1 stores are complete before the arrival is counted
+      wait_stores();
       __syncthreads();
       if (threadIdx.x == 0) add_agent(arrive + 64 * b, 1u);
     } else {
