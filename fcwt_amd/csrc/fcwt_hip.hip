@@ -316,12 +316,21 @@ static void launch_rows_fft(const FwdArgs& f, unsigned batch, hipStream_t st, bo
     launch_rows_fft_t<false>(f, batch, st);
 }
 
+// Transforms per work-group in the four-step passes: as many as fit the LDS,
+// but at most 2^kFwdLog2G, so that one signal's pass spreads over many CUs
+// (the passes are latency-bound: 1 MB moves in each).
+constexpr int kFwdLog2G = 2;
+__host__ __device__ constexpr int fwd_log2g(int log2Hthis, int log2Hother) {
+  return log2Hother < 14 - log2Hthis ? (log2Hother < kFwdLog2G ? log2Hother : kFwdLog2G)
+                                      : (14 - log2Hthis < kFwdLog2G ? 14 - log2Hthis : kFwdLog2G);
+}
+
 // Four-step pass 1: H1-point FFTs over j1 of z[j2 + H2*j1] for G consecutive
 // j2, times W_H^{-j2*k1}, stored to atmp[k1*H2 + j2].
 __global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
   __shared__ float2 lds[kLdsCap];
   const int H1 = 1 << a.log2H1, H2 = 1 << a.log2H2;
-  const int log2G = min(a.log2H2, 14 - a.log2H1);
+  const int log2G = fwd_log2g(a.log2H1, a.log2H2);
   const int G = 1 << log2G;
   const int j20 = blockIdx.x << log2G;
   const int tot = H1 << log2G;
@@ -347,7 +356,7 @@ __global__ __launch_bounds__(kBlock) void k_fwd_a(FwdArgs a) {
 __global__ __launch_bounds__(kBlock) void k_fwd_b(FwdArgs a) {
   __shared__ float2 lds[kLdsCap];
   const int H1 = 1 << a.log2H1, H2 = 1 << a.log2H2;
-  const int log2G = min(a.log2H1, 14 - a.log2H2);
+  const int log2G = fwd_log2g(a.log2H2, a.log2H1);
   const int G = 1 << log2G;
   const int k10 = blockIdx.x << log2G;
   const int tot = H2 << log2G;
@@ -968,8 +977,8 @@ static void launch_fwd_complex(FwdArgs f, int batch, hipStream_t st) {
   if (f.log2H <= 14) {
     launch_rows_fft(f, (unsigned)batch, st, true);
   } else {
-    int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
-    int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
+    int ga = 1 << (f.log2H2 - fwd_log2g(f.log2H1, f.log2H2));
+    int gb = 1 << (f.log2H1 - fwd_log2g(f.log2H2, f.log2H1));
     k_fwd_a<<<dim3(ga, batch), kBlock, 0, st>>>(f);
     k_fwd_b<<<dim3(gb, batch), kBlock, 0, st>>>(f);
   }
@@ -1108,8 +1117,8 @@ static void launch_forward(fcwt_hip_plan p, const float* const* xs, bool complex
     if (f.log2H <= 14) {
       launch_rows_fft(f, ns, st);
     } else {
-      int ga = 1 << (f.log2H2 - std::min(f.log2H2, 14 - f.log2H1));
-      int gb = 1 << (f.log2H1 - std::min(f.log2H1, 14 - f.log2H2));
+      int ga = 1 << (f.log2H2 - fwd_log2g(f.log2H1, f.log2H2));
+      int gb = 1 << (f.log2H1 - fwd_log2g(f.log2H2, f.log2H1));
       k_fwd_a<<<dim3(ga, ns), kBlock, 0, st>>>(f);
       k_fwd_b<<<dim3(gb, ns), kBlock, 0, st>>>(f);
       if (!complex_in) {
