@@ -121,6 +121,21 @@ def test_wide_band_paths(n, teams, monkeypatch):
     assert err.max() <= TOL, f"row {int(np.argmax(err))}: rel-L2 {err.max():.3e}"
 
 
+@pytest.mark.parametrize("fn", [1, 3, 9])
+def test_team_fewer_rows_than_teams(fn):
+    """n = 100,000 with 1, 3 or 9 wide rows: teams without rows, and teams
+    whose rows hold fewer items than the team has members, so most members
+    claim no item and leave at once; every row against the oracle."""
+    x = tones_noise(100_000, seed=31 + fn)
+    sc = O.scales(O.LINFREQS, 1000, 300, 450, fn)
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    assert p.row_split() == {"direct": 0, "team": fn, "two_pass": 0}
+    for _ in range(2):  # the second execute reuses the plan's counters
+        got = run_device(x, sc, 2.0, plan=p)
+        ref = O.cwt_fp64(x, sc, 2.0)
+        assert O.rel_l2_rows(got, ref).max() <= TOL
+
+
 def test_on_bin_cosine_closed_form():
     """n = N, x = cos(2 pi k0 t/N): row i = (1/N)(N/2) mother[idx_i(k0)] e^{2 pi i k0 t/N}."""
     N, k0 = 16384, 1234
