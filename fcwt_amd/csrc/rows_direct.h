@@ -36,8 +36,13 @@ struct DirectCfg {
   static constexpr int NT = G * P2;                // threads
   static constexpr int XS = M + P2;                // LDS stride between phases in the exchange
   static constexpr int OS = G + 1;                 // LDS stride between m rows in the transpose
-  static constexpr int BUF = (P2 > 1 && G * XS > M * OS) ? G * XS : M * OS;
-  static constexpr int LDS = BUF + M /*Z*/ + M /*W_M^{j k1}*/ + R1 * G /*pre-twiddles*/;
+  // Tables: Z[0, M), W_M^{j k1}, the pre-twiddles.  With P2 > 1 the exchange
+  // (two rounds over halves of the phases) and the output transpose (two
+  // rounds over halves of m) reuse their space once they are read.
+  static constexpr int TABLES = M + M + R1 * G;
+  static constexpr int BUFH = (G / 2) * XS > (M / 2) * OS ? (G / 2) * XS : (M / 2) * OS;
+  static constexpr int BUF = P2 > 1 ? 0 : M * OS;  // offset of the tables
+  static constexpr int LDS = P2 > 1 ? (BUFH > TABLES ? BUFH : TABLES) : BUF + TABLES;
 };
 
 template <int LOG2M>
@@ -120,47 +125,71 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
       v[0] = cmul(v[0], base);
 #pragma unroll
       for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[k1 * P2 + j]));
-#pragma unroll
-      for (int k1 = 0; k1 < R1; ++k1) buf[g * C::XS + j * (R1 + 1) + k1] = v[k1];
     }
-    __syncthreads();
-    if (active) {
-      constexpr int U = R1 / P2;
+    __syncthreads();  // the tables are read: the exchange reuses their space
+    // exchange in two rounds, phases [0, G/2) then [G/2, G): g = tid / P2,
+    // so each round is a whole set of wavefronts
+    constexpr int GH = C::G / 2;
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k1 = u * P2 + j;
-        float2 w[P2];
+    for (int h = 0; h < 2; ++h) {
+      const bool mine = active && g >= h * GH && g < (h + 1) * GH;
+      const int gl = g - h * GH;
+      if (mine) {
 #pragma unroll
-        for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * C::XS + jj * (R1 + 1) + k1];
-        dft_reg<P2, 1>(w);
-#pragma unroll
-        for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
+        for (int k1 = 0; k1 < R1; ++k1) buf[gl * C::XS + j * (R1 + 1) + k1] = v[k1];
       }
+      __syncthreads();
+      if (mine) {
+        constexpr int U = R1 / P2;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k1 = u * P2 + j;
+          float2 w[P2];
+#pragma unroll
+          for (int jj = 0; jj < P2; ++jj) w[jj] = buf[gl * C::XS + jj * (R1 + 1) + k1];
+          dft_reg<P2, 1>(w);
+#pragma unroll
+          for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    // transpose to [m][g] in two rounds over m = k1 + R1 k2 (k2 < P2/2, then
+    // the rest), each followed by its 128-byte output runs
+    float2* out = row_out(a, r);
+    constexpr int MH = M / 2, KH = P2 / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (active) {
+        constexpr int U = R1 / P2;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k2 = 0; k2 < KH; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + h * KH + k2];
+      }
+      __syncthreads();
+      for (int e = tid; e < MH * G; e += NT) {
+        const int mm = e / G, gg = e - mm * G;
+        const long long t = (long long)(l0 + gg) + ((long long)(mm + h * MH) << log2L);
+        if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[mm * C::OS + gg]);
+      }
+      if (h == 0) __syncthreads();
+    }
   } else {
-    // single thread per phase: only the phase factor W_N^{q l} = W_{R1 L}^{k l} applies (j = 0)
-  }
-  // transpose through LDS to [m][g]; m = k1 + R1*k2
-  if (active) {
-    if constexpr (P2 > 1) {
-      constexpr int U = R1 / P2;
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int k2 = 0; k2 < P2; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + k2];
-    } else {
+    // single thread per phase: only the phase factor W_N^{q l} = W_{R1 L}^{k l}
+    // applies (j = 0); transpose through LDS to [m][g]
+    if (active) {
 #pragma unroll
       for (int m = 0; m < R1; ++m) buf[m * C::OS + g] = cscale(v[m], a.norm);
     }
-  }
-  __syncthreads();
-  float2* out = row_out(a, r);
-  const int tot = M * G;
-  for (int e = tid; e < tot; e += NT) {
-    int m = e / G, gg = e - m * G;
-    long long t = (long long)(l0 + gg) + ((long long)m << log2L);
-    if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
+    __syncthreads();
+    float2* out = row_out(a, r);
+    const int tot = M * G;
+    for (int e = tid; e < tot; e += NT) {
+      int m = e / G, gg = e - m * G;
+      long long t = (long long)(l0 + gg) + ((long long)m << log2L);
+      if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
+    }
   }
 }
 
