@@ -180,9 +180,17 @@ __device__ __forceinline__ int window_index_d(int q, int nseg, const SegD& s0, c
 // Pass-1 register split of the A-point transform: R1 = min(A, 16) points per
 // thread, P2 = A / R1 threads per transform.
 __host__ __device__ constexpr int team_log2r1(int log2A) { return log2A < 4 ? log2A : 4; }
-// Row stride of the staged band in LDS: A + P2 keeps the reads of one
-// instruction on distinct banks.
-__host__ __device__ constexpr int team_zs_stride(int log2A) { return (1 << log2A) + (1 << (log2A - team_log2r1(log2A))); }
+// Row stride of the staged band in LDS (and, with P2 > 1, of the pass-1
+// exchange rows, which reuse it): at least A + P2, rounded so that the
+// staging writes of one 16-lane LDS cycle fall on distinct banks -- odd when
+// those lanes hold 16 rows ql (Qb >= 16), 2 mod 4 when they hold 8 rows and
+// 2 columns (Qb = 8).  The pass-1 reads of a band row are consecutive or
+// broadcast whatever the stride.
+__host__ __device__ constexpr int team_zs_stride(int log2A, int log2F) {
+  return log2A <= 4 ? (1 << log2A) + 1
+         : 12 - log2F == 3 ? (((1 << log2A) + (1 << (log2A - 4))) | 2) & ~1
+                           : ((1 << log2A) + (1 << (log2A - 4))) | 1;
+}
 // The staged band shares the work-group's LDS buffer with the exchanges and
 // the tile.  With P2 > 1 it sits at the start (pass 1 reads it into registers
 // and waits at a barrier before the exchange overwrites it); with P2 = 1 the
@@ -232,7 +240,7 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
   constexpr int R1 = 1 << LOG2R1;
   constexpr int LOG2P2 = LOG2A - LOG2R1;
   constexpr int P2 = 1 << LOG2P2;
-  constexpr int ZS = team_zs_stride(LOG2A);
+  constexpr int ZS = team_zs_stride(LOG2A, LOG2F);
   constexpr int log2Ph = log2F - LOG2A;  // phases l' per qb
   constexpr int log2Qb = 12 - log2F;
   constexpr int TS = (1 << log2Qb) + 1;  // padded tile row
@@ -252,7 +260,7 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
   } else {
     // 4096/A transforms of A = 16 P2 points, P2 threads each: one round
     static_assert(R1 == 16 && P2 <= 16, "A out of range");
-    constexpr int XS = A + P2;  // exchange [tr][j][k1], j stride R1 + 1
+    constexpr int XS = ZS;  // exchange [tr][j][k1], j stride R1 + 1 (band row ql is transform (ql, 0)'s row)
     const int tr = threadIdx.x >> LOG2P2, j = threadIdx.x & (P2 - 1);
     const int ql = tr >> log2Ph, lp = tr & ((1 << log2Ph) - 1);
     float2 v[R1];
@@ -485,7 +493,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // first is used
       const float maxidx = (float)N - 1.0f;
       const float2* __restrict__ rspec = spec + (size_t)sig * (size_t)(N >> 1);
-      const int zstride = team_zs_stride(log2A);
+      const int zstride = team_zs_stride(log2A, log2F);
       float2* const zs = buf + team_zs_offset(log2A, log2F);
       const int total = A << log2Qb;
       constexpr int kPer = kTeamPoints / 2 / kTeamThreads;
