@@ -284,7 +284,7 @@ __device__ __forceinline__ void cols_pass1(const BigArgs& a, int y, int xblk, fl
 // threads tid of one item (every thread of the work-group reaches its
 // barriers).
 //   thread (g, j): f = f0 + g, qb = j + 16 k (k < 16) in registers
-//   wb[k1][j] = W_256^{j k1}   t2s[g][k] = W_N^{16 k f}   base = c W_N^{j f}
+//   wb[k1][j] = W_256^{j k1}   t2s[k][g] = W_N^{16 k f}   base = c W_N^{j f}
 template <int LOG2F>
 __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, int tid, float2* lds) {
   constexpr int XS = 16 * 17;  // exchange [k1][g][j]: k1 * XS + g * 17 + j
@@ -298,7 +298,7 @@ __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, in
   {
     const int k1 = tid >> 4, jj = tid & 15;
     wb[tid] = a.k.tw((unsigned)(k1 * jj) << LOG2F);
-    const int gg = tid >> 4, k = tid & 15;
+    const int k = tid >> 4, gg = tid & 15;  // [k][g]: the 16 lanes of one k read consecutive entries
     t2s[tid] = a.k.tw((unsigned)((16 * k * (f0 + gg)) & (N - 1)));
   }
   const float2 base = cscale(a.k.tw((unsigned)(j * (f0 + g))), a.k.norm);
@@ -308,7 +308,7 @@ __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, in
   for (int k = 0; k < 16; ++k) v[k] = gload(U + ((size_t)(j + 16 * k) << LOG2F) + f0 + g);
   __syncthreads();  // tables
 #pragma unroll
-  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], t2s[g * 16 + k]);
+  for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], t2s[k * 16 + g]);
   dft_reg<16, 1>(v);
   v[0] = cmul(v[0], base);
 #pragma unroll
