@@ -274,15 +274,15 @@ __global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
   const int nx = fwd_len(a, blockIdx.y);
   float2* spec = a.spec + blockIdx.y * a.spec_stride;
   row_tw_tables<LOG2H>(s);
-  for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[j] = packed(x, nx, j);
+  for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[row_pad(j)] = packed(x, nx, j);
   __syncthreads();
   row_fft<LOG2H, -1, DERIVED>(s);
   if (a.complex_in) {
-    for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[k];
+    for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[row_pad(k)];
     return;
   }
   for (int k = threadIdx.x; k < H; k += kRowFftThreads) {
-    float2 zk = s.data[k], zc = s.data[(H - k) & (H - 1)];
+    float2 zk = s.data[row_pad(k)], zc = s.data[row_pad((H - k) & (H - 1))];
     spec[k] = real_post(zk, zc, a.tw.get((unsigned)k, -1.0f));
   }
 }

@@ -17,12 +17,16 @@ namespace fcwt_hip {
 
 constexpr int kRowFftThreads = 1024;
 
+// LDS index of point i: one pad word per 16 points keeps the stride-16 and
+// stride-4 writes of the first passes on distinct banks.
+__device__ __forceinline__ int row_pad(int i) { return i + (i >> 4); }
+
 template <int LOG2H>
 struct RowFftLds {
   static constexpr int H = 1 << LOG2H;
   static constexpr int T1N = H < 128 ? H : 128;
   static constexpr int T2N = H / T1N;
-  float2 data[H];
+  float2 data[H + (H >> 4)];  // point i at row_pad(i)
   float2 t1[T1N];
   float2 t2[T2N];
 };
@@ -70,7 +74,7 @@ __device__ __forceinline__ void row_fft_pass(RowFftLds<LOG2H>& s) {
     if (GROUPS % kRowFftThreads == 0 || g < GROUPS) {
       const int jj = g & (Ns - 1);
 #pragma unroll
-      for (int k = 0; k < R; ++k) v[u][k] = s.data[g + k * GROUPS];
+      for (int k = 0; k < R; ++k) v[u][k] = s.data[row_pad(g + k * GROUPS)];
       if (LOG2NS > 0) {
         constexpr int SH = LOG2H - LOG2NS - LOG2R;
         if constexpr (R == 16 && DERIVED) {
@@ -105,7 +109,7 @@ __device__ __forceinline__ void row_fft_pass(RowFftLds<LOG2H>& s) {
       const int jj = g & (Ns - 1);
       const int d = ((g >> LOG2NS) << (LOG2NS + LOG2R)) + jj;
 #pragma unroll
-      for (int k = 0; k < R; ++k) s.data[d + k * Ns] = v[u][k];
+      for (int k = 0; k < R; ++k) s.data[row_pad(d + k * Ns)] = v[u][k];
     }
   }
   __syncthreads();
