@@ -121,6 +121,23 @@ def test_wide_band_paths(n, teams, monkeypatch):
     assert err.max() <= TOL, f"row {int(np.argmax(err))}: rel-L2 {err.max():.3e}"
 
 
+@pytest.mark.parametrize("nbuf", [2, 3, 12])
+def test_team_buffer_counts(nbuf, monkeypatch):
+    """The XCD-team kernel with the fewest scratch buffers (2 and 3: pass 2
+    one step behind pass 1, no slack) and the most (12), through the
+    environment override: same rows as the default, within the bar."""
+    monkeypatch.setenv("FCWT_HIP_TEAM_BUFFERS", str(nbuf))
+    x = tones_noise(100_000, seed=40 + nbuf)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 120)
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    assert p.row_split()["team"] > 0
+    for _ in range(2):
+        got = run_device(x, sc, 2.0, plan=p)
+        rows = sorted(set(range(0, 120, 7)) | {119})
+        ref = O.cwt_fp64(x, sc, 2.0, rows=rows)
+        assert O.rel_l2_rows(got[rows], ref).max() <= TOL
+
+
 @pytest.mark.parametrize("fn", [1, 3, 9])
 def test_team_fewer_rows_than_teams(fn):
     """n = 100,000 with 1, 3 or 9 wide rows: teams without rows, and teams
