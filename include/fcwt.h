@@ -18,6 +18,8 @@
 
 #include <complex>
 #include <cstdio>
+#include <iostream>
+#include <memory>
 #include <string>
 
 #include "fcwt_hip.h"
@@ -25,6 +27,10 @@
 #define PI 3.14159265358979323846264338327950288419716939937510582097494459072381640628620899862803482534211706798f
 #define sqrt2PI 2.50662827463100050241576528f
 #define IPI4 0.75112554446f
+
+// The reference header opens namespace std for its users (fcwt.h:63), and
+// code written against it uses complex<float>, vector, cout unqualified.
+using namespace std;
 
 enum SCALETYPE { FCWT_LINSCALES, FCWT_LOGSCALES, FCWT_LINFREQS };
 
