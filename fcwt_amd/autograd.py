@@ -51,7 +51,10 @@ class _CWTFunction(torch.autograd.Function):
         else:
             for b in range(B):
                 plan.execute_device(rows[b].data_ptr(), out[b].data_ptr(), st, complex_input=cplx)
+        if op.check_errors:
+            op.check()
         ctx.plan = plan
+        ctx.op = op
         ctx.cplx = cplx
         ctx.in_dtype = x.dtype
         return out.reshape(*batch, plan.fn, plan.n)
@@ -66,20 +69,37 @@ class _CWTFunction(torch.autograd.Function):
         for b0 in range(0, B, 65535):
             cnt = min(65535, B - b0)
             plan.adjoint_batch_device(g[b0].data_ptr(), xbar[b0].data_ptr(), cnt, st, complex_input=cplx)
+        if ctx.op.check_errors:
+            ctx.op.check()
         shape = grad_out.shape[:-2] + (plan.n,)
         return xbar.reshape(shape).to(ctx.in_dtype), None
 
 
 class CWT(torch.nn.Module):
     """The Morlet CWT of signals of length ``n`` at ``scales`` as a
-    differentiable module; output (..., fn, n) complex64."""
+    differentiable module; output (..., fn, n) complex64.
 
-    def __init__(self, n: int, scales, fb: float = 2.0, normalize: bool = True, threads: int = 1):
+    The launches are asynchronous.  If an in-kernel synchronisation of the
+    wide-band kernel fails, that launch's wide-band rows come out as NaN and
+    ``check()`` raises ``FcwtError`` (once); later calls run normally.
+    ``check_errors=True`` runs ``check()`` after every forward and backward
+    (it waits for the device)."""
+
+    def __init__(self, n: int, scales, fb: float = 2.0, normalize: bool = True, threads: int = 1,
+                 check_errors: bool = False):
         super().__init__()
         self._scales = np.ascontiguousarray(scales, np.float32)
         self._args = (float(fb), int(threads), bool(normalize))
         self.plan = Plan(int(n), self._scales, float(fb), threads, normalize)
         self._batches = {}
+        self.check_errors = bool(check_errors)
+
+    def check(self) -> None:
+        """Waits for the device; raises FcwtError if a launch of this
+        module's plans failed since the last check."""
+        self.plan.check()
+        for b in self._batches.values():
+            b.check()
 
     def batch_for(self, B: int) -> Batch:
         """The batch plan for B signals of this length (built once per B)."""

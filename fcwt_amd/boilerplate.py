@@ -6,18 +6,38 @@ import numpy as np
 from .api import FCWT, FCWT_LINFREQS, FCWT_LOGSCALES, Morlet, Scales
 
 
+# FCWT objects kept between calls of ``cwt``, one per (nthreads, fast, norm):
+# each holds the device plan of its last (n, scales), so repeated calls on
+# signals of one shape build the plan (device tables, schedule, staging
+# buffers) once.  The reference builds its FFTW plans on every call
+# (fcwt.cpp:486-502), which costs little on the CPU; a device plan does not.
+_FCWT_CACHE = {}
+
+
+def clear_plan_cache():
+    """Drops the plans ``cwt`` keeps, and their device memory."""
+    for f in _FCWT_CACHE.values():
+        if f._plan is not None:
+            f._plan.close()
+    _FCWT_CACHE.clear()
+
+
 def cwt(input, fs, f0, f1, fn, nthreads=1, scaling="lin", fast=False, norm=True):
     """``fcwt.cwt`` (boilerplate.py:5-34): Morlet(2.0), LINFREQS for
     ``scaling="lin"`` else LOGSCALES; returns ``(freqs, output)`` with output
-    a (fn, n) complex64 array computed on the GPU."""
+    a (fn, n) complex64 array computed on the GPU.  The device plan of the
+    last call is reused while n and the scales are unchanged."""
     input = np.asarray(input)
     if input.ndim > 1:
         raise ValueError("Input must be a vector")
     if input.dtype != np.float32:
         input = input.astype(np.float32)
-    morl = Morlet(2.0)
+    key = (int(nthreads), bool(fast), bool(norm))
+    fcwt = _FCWT_CACHE.get(key)
+    if fcwt is None:
+        fcwt = _FCWT_CACHE[key] = FCWT(Morlet(2.0), nthreads, fast, norm)
+    morl = fcwt.wavelet
     scales = Scales(morl, FCWT_LINFREQS if scaling == "lin" else FCWT_LOGSCALES, fs, f0, f1, fn)
-    fcwt = FCWT(morl, nthreads, fast, norm)
     output = np.zeros((fn, input.size), dtype=np.complex64)
     freqs = np.zeros((fn,), dtype=np.float32)
     fcwt.cwt(input, scales, output)

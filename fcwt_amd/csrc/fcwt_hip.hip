@@ -541,7 +541,6 @@ struct fcwt_hip_plan_s {
   // XCD-team path for wide bands (rows_team.h)
   bool team_capable = false;
   int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, seq_stride = 0;
-  long team_spin = kTeamSpin;  // FCWT_HIP_TEAM_SPIN lowers it to force a timed-out wait in tests
   std::vector<int> team_rows, team_seq, team_blocks;
   std::vector<TeamBlk> team_desc;  // team_seq with each block's row fields
   TeamBlk* d_team_desc = nullptr;
@@ -906,8 +905,9 @@ static int check_device(int* dev) {
   return FCWT_HIP_OK;
 }
 
-// The team path launches kTeamWgPerCu work-groups per CU cooperatively and
-// forms teams from the work-groups each XCD receives: it needs the full 256-CU device in
+// The team path launches kTeamWgPerCu work-groups per CU (an ordinary
+// launch) and makes one team of the work-groups each XCD receives, whose
+// scratch stays in that XCD's L2: it is meant for the full 256-CU device in
 // one partition.  FCWT_HIP_NO_TEAMS=1 in the environment turns it off.
 static const void* team_kernel(int log2F) {
   switch (log2F) {
@@ -925,7 +925,7 @@ static bool team_capable(int dev, int log2N) {
   if (!kern) return false;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
-  if (prop.multiProcessorCount != 256 || !prop.cooperativeLaunch) return false;
+  if (prop.multiProcessorCount != 256) return false;
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kTeamThreads, 0) != hipSuccess) return false;
   return per_cu >= kTeamWgPerCu;
@@ -1157,6 +1157,18 @@ static void launch_cols(int log2F, const BigArgs& b1, int n1, const BigArgs& b2,
   }
 }
 
+// Polls before a team wait is declared failed: kTeamSpin, or fewer from
+// FCWT_HIP_TEAM_SPIN (0: every wait fails at once), read at every launch so
+// that a test can force one failed launch and then run the same plan
+// normally.
+static long team_spin() {
+  if (const char* e = std::getenv("FCWT_HIP_TEAM_SPIN")) {
+    const long v = std::atol(e);
+    if (v >= 0) return std::min(v, kTeamSpin);
+  }
+  return kTeamSpin;
+}
+
 static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
                       hipStream_t st) {
   const bool multi = p->nsig > 1;
@@ -1194,16 +1206,14 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     ta.team_blocks = p->d_team_blocks;
     ta.seq_stride = p->seq_stride;
     ta.nteams = p->nteams;
-    ta.team_size = p->team_size;
     ta.log2F = p->team_log2F;
     ta.nbuf = p->team_nbuf;
-    ta.spin = p->team_spin;
+    ta.spin = team_spin();
     ta.scratch = p->d_team_scratch;
     ta.sync = p->d_team_sync;
     ta.error = p->d_error;
     void* args[] = {&ta};
-    HIP_TRY(hipLaunchCooperativeKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args,
-                                       0, st));
+    HIP_TRY(hipLaunchKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0, st));
   }
   if (!p->cols_batches.empty()) {
     const int lf = p->cols_log2F;
@@ -1359,10 +1369,6 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   p->normalize = normalize != 0;
   p->flags = flags;
   p->team_capable = team_capable(dev, log2N);
-  if (const char* e = std::getenv("FCWT_HIP_TEAM_SPIN")) {
-    const long v = std::atol(e);
-    if (v >= 1) p->team_spin = std::min(v, kTeamSpin);
-  }
   for (int s = 0; s < nsig; ++s) {
     p->sig_row0.push_back((int)p->scales.size());
     p->sig_n.push_back(lengths[s]);
@@ -1473,7 +1479,7 @@ static int read_error_word(fcwt_hip_plan p) {
   if (h != 0) {
     unsigned zero = 0;
     HIP_TRY(hipMemcpy(p->d_error, &zero, sizeof(unsigned), hipMemcpyHostToDevice));
-    return fail(FCWT_HIP_EDEVICE, h & 2u ? "wide-band kernel: work-groups were not spread evenly over the XCDs"
+    return fail(FCWT_HIP_EDEVICE, h & 2u ? "wide-band kernel: an XCD's team left rows unclaimed"
                                          : "wide-band kernel: a team wait timed out");
   }
   return FCWT_HIP_OK;

@@ -19,15 +19,23 @@
 // counter (an item claims its member's next one when it starts, so that the
 // claim's latency is hidden), which balances uneven items across members.
 // Every item depends only on earlier ones, and the smallest unfinished item
-// is always some member's current item, so the walk cannot deadlock while
-// all members are resident (cooperative launch).  A pass-1 item stores normally, waits
+// is always some running member's current item (items are claimed in order,
+// and only by running work-groups), so the walk progresses with however many
+// members are resident: the launch is an ordinary one, with no grid-wide
+// barrier and no cooperative-launch resource.  A work-group's team is the XCD
+// it runs on; when the last work-group exits it checks that every team's
+// sequence was claimed to its end (an XCD that received no work-group would
+// leave its rows unwritten; that is reported as an error).  A pass-1 item stores normally, waits
 // for its stores (s_waitcnt), then bumps its buffer's arrival counter; a
 // pass-2 item waits for the row's count and reads with device-scope loads,
 // which bypass the CU's L1 and are served by the shared L2, then bumps the
 // buffer's done counter, which the next pass 1 into that buffer waits for
 // (after staging its band in LDS, so that those loads overlap the wait).
 // nbuf buffers per team give the sequence slack on both sides.  Every wait is
-// bounded: on timeout the kernel records an error and every work-group exits.
+// bounded: on timeout the kernel records an error in the launch's abort word
+// (every other wait of the launch then gives up too) and in the plan's
+// report word, every work-group exits, and the last one to exit fills the
+// launch's team rows with NaN.  The next launch starts with a clear abort word.
 //
 // Twiddle tables in LDS are laid out so that the lanes of one instruction
 // read consecutive or identical entries (no bank conflicts): pass-2 factors
@@ -70,18 +78,19 @@ struct TeamArgs {
   const int* team_blocks;  // blocks in team t's sequence
   int seq_stride;
   int nteams;              // teams on the whole device (one per XCD)
-  int team_size;           // work-groups per team
   int log2F;               // F = N / 256 (the kernel's template parameter)
   int nbuf;                // scratch buffers per team (2..kTeamBuffers)
   long spin;               // polls before a wait is declared failed (kTeamSpin unless a test lowers it)
   float2* scratch;         // [team][nbuf][N]
   unsigned* sync;          // zeroed before launch; see below
-  unsigned* error;         // set to a nonzero code on a failed wait or team formation
+  unsigned* error;         // sticky report: ORed with the launch's abort code, cleared by the host when read
 };
 // sync layout (unsigned, 32 per 128-byte line):
-//   [0..7] work-groups per XCD, [8] grid arrival counter,
+//   [32] this launch's abort code, [33] work-groups that have finished,
 //   team t: arrive[b] at 64 * (1 + t * kTeamBuffers + b), done[b] 32 later;
 //   team t's next-item counter at 64 * (1 + nteams * kTeamBuffers + t).
+constexpr int kSyncAbort = 32;
+constexpr int kSyncExited = 33;
 
 // Load through the constant address space. The schedule arrays are written
 // before the launch and never during it, so a uniform address becomes a
@@ -119,22 +128,68 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_sched_barrier(0);  // and keeps the register DFTs on either side apart
 }
 
-// Thread 0 waits until *p >= target (at most `spin` polls); the result is
-// broadcast.
-__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, int* s_flag, unsigned* error,
-                                          long spin) {
+// Records a failure: the launch's abort word (every wait of this launch
+// gives up on it) and the sticky report word the host reads.
+__device__ __forceinline__ void team_fail(unsigned* abort, unsigned* error, unsigned code) {
+  atomicOr(abort, code);
+  atomicOr(error, code);
+}
+
+// Thread 0 waits until *p >= target (at most `spin` polls, and not after the
+// launch has aborted); the result is broadcast.  spin == 0 fails every wait
+// at once: the host sets it only to test the failure path.
+__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, int* s_flag, unsigned* abort,
+                                          unsigned* error, long spin) {
   if (threadIdx.x == 0) {
     long s = 0;
-    while (ld_agent(p) < target && ld_agent(error) == 0 && s < spin) {
+    while (ld_agent(p) < target && ld_agent(abort) == 0 && s < spin) {
       __builtin_amdgcn_s_sleep(1);
       ++s;
     }
-    bool ok = ld_agent(p) >= target;
-    if (!ok) atomicOr(error, 1u);
+    bool ok = spin > 0 && ld_agent(p) >= target;
+    if (!ok) team_fail(abort, error, 1u);
     *s_flag = ok;
   }
   __syncthreads();
   return *s_flag != 0;
+}
+
+// A work-group's exit.  Its stores are complete before it counts itself
+// out, so the last work-group to finish knows that no other writes any more.
+// It checks that every team claimed its whole sequence, and when the launch
+// has aborted (or a team's sequence was left unclaimed) it fills every team
+// row of the launch with NaN: a failed launch never leaves rows that look
+// valid.
+__device__ __noinline__ void team_exit(unsigned* sync, unsigned* error, const TeamBlk* team_desc,
+                                      const int* team_blocks, int seq_stride, int nteams, int log2I, float2* out1,
+                                      float2* const* outs, int* s_flag) {
+  wait_stores();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __atomic_thread_fence(__ATOMIC_RELEASE);
+    const unsigned done = __hip_atomic_fetch_add(sync + kSyncExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    bool fill = false;
+    if (done + 1 == gridDim.x) {
+      for (int t = 0; t < nteams && ld_agent(sync + kSyncAbort) == 0; ++t) {
+        const unsigned claimed = ld_agent(sync + 64 * (1 + nteams * kTeamBuffers + t));
+        if (claimed < ((unsigned)team_blocks[t] << log2I)) team_fail(sync + kSyncAbort, error, 2u);
+      }
+      fill = ld_agent(sync + kSyncAbort) != 0;
+    }
+    *s_flag = fill;
+  }
+  __syncthreads();
+  if (!*s_flag) return;
+  const float qnan = __int_as_float(0x7fc00000);
+  for (int t = 0; t < nteams; ++t) {
+    const int nb = team_blocks[t];
+    for (int s = 0; s < nb; ++s) {
+      const TeamBlk d = team_desc[(size_t)t * seq_stride + s];
+      if ((d.sq & 1) == 0) continue;  // each row once: its pass-2 block
+      float2* out = (outs ? outs[d.sig] : out1) + d.out_off;
+      for (int e = threadIdx.x; e < d.n; e += blockDim.x) out[e] = make_float2(qnan, qnan);
+    }
+  }
 }
 
 // Index (int)fminf(N-1, step*q) into the mother table for the segment holding
@@ -391,7 +446,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   float2* const post = tm_post;
   float2* const t2s = tm_t2;
   float2* const wb = tm_wb;
-  __shared__ int s_rank, s_flag, s_next;
+  __shared__ int s_flag, s_next;
   // plain copies of the arguments (never take the address of the kernel argument)
   const float2* __restrict__ spec = ta.k.spec;
   const float* __restrict__ mother = ta.k.mother;
@@ -399,29 +454,14 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   const Twiddles tw = ta.k.tw;
   const int N = ta.k.N;
   const float norm = ta.k.norm;
-  const int team_size = ta.team_size;
   constexpr int log2F = LOG2F;
   unsigned* const sync = ta.sync;
   unsigned* const error = ta.error;
 
-  // team formation: rank among the work-groups of this XCD
-  const int xcc = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;
-  if (threadIdx.x == 0) {
-    s_rank = (int)atomicAdd(&sync[xcc], 1u);
-    add_agent(&sync[8], 1u);
-  }
-  __syncthreads();
+  unsigned* const abort = sync + kSyncAbort;
   const long spin = ta.spin;
-  if (!team_wait(&sync[8], gridDim.x, &s_flag, error, spin)) return;
-  if (threadIdx.x == 0) {
-    bool ok = true;
-    for (int x = 0; x < 8; ++x) ok &= ld_agent(&sync[x]) == (unsigned)team_size;
-    if (!ok) atomicOr(error, 2u);
-    s_flag = ok;
-  }
-  __syncthreads();
-  if (!s_flag) return;
-  const int team = xcc;
+  // the team: the XCD this work-group runs on
+  const int team = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;
   constexpr int F = 1 << log2F;
   constexpr int log2Qb = 12 - log2F;
   constexpr int log2I = log2F - 4;  // items per pass per row: F/16
@@ -521,7 +561,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // the buffer's previous row has been read by every pass-2 item (the
       // wait ends in a barrier, which also publishes the staged band)
       if (gen > 0) {
-        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, error, spin)) return;
+        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, abort, error, spin)) break;
       } else {
         __syncthreads();
       }
@@ -540,7 +580,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // W_N^{j f} times the row normalisation
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
-      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, error, spin)) return;
+      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, abort, error, spin)) break;
       team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
     }
     // publishes the next item, and every thread is done with this item's LDS
@@ -548,6 +588,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(s_next);
   }
+  team_exit(sync, error, ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, log2I, ta.k.out, ta.k.outs, &s_flag);
 }
 
 }  // namespace fcwt_hip

@@ -32,14 +32,14 @@ def test_allocation_failure_is_reported_and_recovers(monkeypatch):
 
 
 def test_timed_out_wait_is_reported_once_and_recovers(monkeypatch):
-    """A spin limit of one poll: the first work-group to reach the team
-    formation wait finds the others missing, records the error, and every
-    work-group exits; plan_check reports it, and a later check is clean."""
+    """A spin limit of zero polls: the first team wait of the wide-band
+    kernel fails, records the error, and every work-group exits;
+    plan_check reports it, and a later check is clean."""
     import torch
 
     x = tones_noise(100_000, seed=4)
     sc = O.scales(O.LOGSCALES, 1000, 1, 450, 300)
-    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "1")
+    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "0")
     p = fcwt_amd.Plan(x.size, sc, 2.0)
     assert p.row_split()["team"] > 0
     dx = torch.from_numpy(x).cuda()
@@ -82,3 +82,48 @@ def test_library_before_torch_shares_one_runtime():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=110)
     assert r.returncode == 0 and "ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+
+
+def test_failed_launch_is_not_sticky_and_surfaces_through_autograd(monkeypatch):
+    """One forced failure on a plan that stays in use: the failed launch's
+    wide-band rows are NaN (never stale values that look valid), check()
+    raises once, and the next forward of the same module and plan -- one
+    signal through the plan, two through the batch plan -- is correct."""
+    import torch
+
+    from fcwt_amd.autograd import CWT
+
+    n = 60_000
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 200)
+    op = CWT(n, sc, 2.0)
+    split = op.plan.row_split()
+    assert split["team"] > 0 and split["direct"] > 0
+    x = torch.from_numpy(np.stack([tones_noise(n, seed=s) for s in (5, 6)])).cuda()
+    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "0")
+    for xin in (x[0], x):
+        W = op(xin)
+        torch.cuda.synchronize()
+        with pytest.raises(nat.FcwtError) as e:
+            op.check()
+        assert "timed out" in str(e.value)
+        finite = torch.isfinite(W.reshape(-1, sc.size, n)).all(dim=-1).cpu().numpy()
+        assert not finite.all(axis=1).any()  # every signal has NaN rows (its team rows)
+        assert finite.any()                  # the direct rows were still written
+        op.check()                           # reported once
+    monkeypatch.delenv("FCWT_HIP_TEAM_SPIN")
+    rows = [0, 60, 120, 199]
+    for xin in (x[0], x):
+        W = op(xin).reshape(-1, sc.size, n)
+        op.check()
+        for b in range(W.shape[0]):
+            got = W[b].cpu().numpy()
+            assert np.isfinite(got).all()
+            ref = O.cwt_fp64(x[b].cpu().numpy(), sc, 2.0, rows=rows)
+            assert O.rel_l2_rows(got[rows], ref).max() <= 1e-5
+    # opt-in checking after every forward raises at the call itself
+    op.check_errors = True
+    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "0")
+    with pytest.raises(nat.FcwtError):
+        op(x[0])
+    monkeypatch.delenv("FCWT_HIP_TEAM_SPIN")
+    op(x[0])

@@ -150,6 +150,12 @@ def test_plan(tmp_path, monkeypatch, capsys):
     fs, n = 64, 10000
     x = np.sin(2 * np.pi * 3.3 * (np.arange(n) / fs), dtype="single")
     morl = Morlet(2.0)
+    # the untuned result, before any tuning enters this process's table (the
+    # table is process-wide, as FFTW's wisdom is in the reference)
+    scs = Scales(morl, FCWT_LOGSCALES, fs, 1, 32, 300)
+    xx = np.resize(x, 30_000)
+    b = np.zeros((300, 30_000), np.complex64)
+    FCWT(morl, 8, False, False).cwt(xx, scs, b)
     fc = FCWT(morl, 1, True, False)
     fc.create_FFT_optimization_plan(2048, "FFTW_MEASURE")
     assert os.path.isfile("n2048_t1.fcwt_hip")
@@ -163,15 +169,53 @@ def test_plan(tmp_path, monkeypatch, capsys):
     assert "cols_mb" in text and "path" in text
     # cwt with optimisation schemes reads the file (no warning) and agrees
     # with the untuned result (the tuned path may sum in another order)
-    scs = Scales(morl, FCWT_LOGSCALES, fs, 1, 32, 300)
     a = np.zeros((300, 30_000), np.complex64)
-    xx = np.resize(x, 30_000)
     capsys.readouterr()
     fc.cwt(xx, scs, a)
     assert "WARNING" not in capsys.readouterr().out
-    b = np.zeros_like(a)
-    FCWT(morl, 8, False, False).cwt(xx, scs, b)
     err = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(b, axis=1), 1e-30)
     assert err.max() <= 2e-6, err.max()
     FCWT(morl, 3, True, False).cwt(xx, scs, b)  # no file for T = 3
     assert "WARNING: Optimization scheme 'n32768_t3.fcwt_hip' was not found" in capsys.readouterr().out
+
+
+@pytest.mark.gpu
+def test_tuning_file_changes_the_schedule(tmp_path):
+    """A tuning file that routes the wide rows of N = 2^15 to the column path
+    changes the plan's schedule (row split) but not its result beyond
+    summation order.  Run in a child process, so that this process's tuning
+    table stays untouched."""
+    import subprocess
+    import sys
+
+    code = r"""
+import numpy as np, os, torch, fcwt_amd
+torch.zeros(1, device="cuda:0")
+from fcwt_amd import _native as nat
+from oracle import oracle as O
+from tests.gpu_util import run_device, tones_noise
+n = 30_000
+sc = O.scales(O.LOGSCALES, 1000, 1, 450, 200)
+x = tones_noise(n, seed=12)
+p0 = fcwt_amd.Plan(n, sc, 2.0)
+s0 = p0.row_split()
+ref = run_device(x, sc, 2.0, plan=p0)
+path = os.path.join(os.environ["TUNE_DIR"], "n32768_t1.fcwt_hip")
+open(path, "w").write(chr(10).join(["fcwt_hip tuning 1", "arch gfx950", "N 32768", "threads 1", "effort 1", "team_buffers 3", "cols_mb 64", "path cols", ""]))
+assert nat.lib().fcwt_hip_tune_load(path.encode()) == 0
+p1 = fcwt_amd.Plan(n, sc, 2.0)
+s1 = p1.row_split()
+got = run_device(x, sc, 2.0, plan=p1)
+assert s0["team"] > 0 and s1["team"] == 0 and s1["two_pass"] == s0["team"], (s0, s1)
+err = O.rel_l2_rows(got, ref).max()
+assert err <= 2e-6, err
+rows = [0, 100, 199]
+assert O.rel_l2_rows(got[rows], O.cwt_fp64(x, sc, 2.0, rows=rows)).max() <= 1e-5
+print("ok", s0, s1, err)
+"""
+    import os
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, TUNE_DIR=str(tmp_path))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, env=env, capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
