@@ -51,6 +51,9 @@ SIGNATURES = {
     "fcwt_hip_plan_row_split": (ctypes.c_int, [_plan_t, _c_int_p, _c_int_p, _c_int_p]),
     "fcwt_hip_cwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_cwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fcwt_hip_cwt_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_float_p, ctypes.c_int,
+                                                   ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                                   ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_ccwt": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_ccwt_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fcwt_hip_adjoint_device": (ctypes.c_int, [_plan_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -48,6 +49,12 @@ constexpr int kMaxLog2B = 10;  // writer FFT size cap: G * B <= kLdsCap with G >
 constexpr double kStaleFloor = 1e-12;
 constexpr size_t kScratchBytes = size_t(256) << 20;
 constexpr size_t kColsScratchBytes = size_t(128) << 20;  // column path: scratch slots per batch
+// The no-allocation entry point: its plans live in a static device array of
+// kNoAllocBytes, hold signals of at most kNoAllocMaxN padded samples, and
+// give the two-pass kernels at most kNoAllocScratchBudget of scratch.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxN = 1 << 17;
+constexpr size_t kNoAllocScratchBudget = size_t(16) << 20;
 constexpr int kKinds = 8;  // kernel kinds timed by fcwt_hip_plan_profile
 
 struct SegD {
@@ -603,7 +610,14 @@ struct fcwt_hip_plan_s {
   size_t stage_bytes = 0;
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: the caller's stream (no-allocation entry point)
   size_t bytes = 0;
+  // arena mode (fcwt_hip_cwt_device_noalloc): device memory is carved from
+  // the library's static device array instead of hipMalloc
+  char* arena = nullptr;
+  size_t arena_cap = 0;
+  size_t wide_budget = kScratchBytes;  // generic two-pass kernels' scratch
+  size_t cols_budget = 0;              // column path, per batch of rows (0: tuned or default)
   // per-kernel timing (fcwt_hip_plan_profile): event pairs around launches
   struct Timed {
     int kind;
@@ -675,6 +689,12 @@ static void free_plan(fcwt_hip_plan p) {
     if (p->h_stage[k]) (void)hipHostFree(p->h_stage[k]);
     if (p->stage_ev[k]) (void)hipEventDestroy(p->stage_ev[k]);
   }
+  if (p->arena) {  // nothing was allocated: the memory belongs to the static array
+    if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
+    (void)hipSetDevice(cur);
+    delete p;
+    return;
+  }
   void* ptrs[] = {p->d_mother, p->d_tw1, p->d_tw2, p->d_spec, p->d_ztmp, p->d_atmp, p->d_segs,
                   p->d_rows,   p->d_items, p->d_scratch, p->d_x, p->d_out,
                   p->d_team_desc, p->d_team_blocks, p->d_team_scratch, p->d_team_sync, p->d_error,
@@ -682,7 +702,7 @@ static void free_plan(fcwt_hip_plan p) {
                   p->d_sig_n, p->d_xs, p->d_outs, p->d_cols_rows, p->d_cols_scratch, p->d_xt};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  if (p->stream) (void)hipStreamDestroy(p->stream);
+  if (p->stream && p->own_stream) (void)hipStreamDestroy(p->stream);
   (void)hipSetDevice(cur);
   delete p;
 }
@@ -699,6 +719,17 @@ static size_t device_budget() {
 template <class T>
 static int dalloc(fcwt_hip_plan p, T** ptr, size_t count) {
   if (count == 0) count = 1;
+  if (p->arena) {  // 256-byte aligned pieces of the static array, no allocation
+    const size_t at = (p->bytes + 255) & ~size_t(255);
+    if (at + count * sizeof(T) > p->arena_cap) {
+      *ptr = nullptr;
+      return fail(FCWT_HIP_EUNSUPPORTED, "this plan needs more than the " + std::to_string(p->arena_cap >> 20) +
+                                             " MiB of device memory the no-allocation entry point holds");
+    }
+    *ptr = reinterpret_cast<T*>(p->arena + at);
+    p->bytes = at + count * sizeof(T);
+    return FCWT_HIP_OK;
+  }
   if (p->bytes + count * sizeof(T) > device_budget()) {
     *ptr = nullptr;
     return fail(FCWT_HIP_ENOMEM, "device allocation of " + std::to_string(count * sizeof(T)) +
@@ -839,6 +870,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       long v = std::atol(e);
       if (v >= 1 && v <= 4096) budget = (size_t)v << 20;
     }
+    if (p->cols_budget) budget = p->cols_budget;
     p->cols_slots = (int)std::max<size_t>(1, std::min<size_t>(cols.size(), budget / per_row));
     p->cols_rows = cols;
     for (size_t b0 = 0; b0 < cols.size(); b0 += p->cols_slots) {
@@ -869,7 +901,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       return fail(FCWT_HIP_EUNSUPPORTED, "padded size N = 2^" + std::to_string(p->log2N) +
                                              " exceeds this build's limit of 2^24 for wide scales");
     size_t per_row = sizeof(float2) * (size_t)p->N;
-    p->scratch_slots = (int)std::max<size_t>(1, std::min<size_t>(wide.size(), kScratchBytes / per_row));
+    p->scratch_slots = (int)std::max<size_t>(1, std::min<size_t>(wide.size(), p->wide_budget / per_row));
     for (size_t b0 = 0; b0 < wide.size(); b0 += p->scratch_slots) {
       size_t b1 = std::min(wide.size(), b0 + p->scratch_slots);
       fcwt_hip_plan_s::Batch bt;
@@ -1331,7 +1363,8 @@ int fcwt_hip_describe_rows(int n, const float* sc, int fn, float fb, int threads
 // lengths[s] samples and fns[s] scales at scales[s].  One-signal plans are
 // the public fcwt_hip_plan_create; batches hold one plan per padded size.
 static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const float* const* sc, const int* fns,
-                       float fb, int threads, int normalize, unsigned flags, const char* what) {
+                       float fb, int threads, int normalize, unsigned flags, const char* what,
+                       char* arena = nullptr, size_t arena_cap = 0, hipStream_t stream = nullptr) {
   if (!out) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null plan pointer");
   *out = nullptr;
   if (nsig < 1 || !lengths || !sc || !fns) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
@@ -1369,6 +1402,14 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   p->normalize = normalize != 0;
   p->flags = flags;
   p->team_capable = team_capable(dev, log2N);
+  if (arena) {  // everything in the static array, on the caller's stream
+    p->arena = arena;
+    p->arena_cap = arena_cap;
+    p->own_stream = false;
+    p->stream = stream;
+    p->wide_budget = kNoAllocScratchBudget;
+    p->cols_budget = kNoAllocScratchBudget;
+  }
   for (int s = 0; s < nsig; ++s) {
     p->sig_row0.push_back((int)p->scales.size());
     p->sig_n.push_back(lengths[s]);
@@ -1381,7 +1422,7 @@ static int create_plan(fcwt_hip_plan* out, int nsig, const int* lengths, const f
   morlet_mother(fb, p->N, mother_host.data());
   TRY(build_schedule(p, mother_host));
 
-  if (hipStreamCreate(&p->stream) != hipSuccess) {
+  if (!p->arena && hipStreamCreate(&p->stream) != hipSuccess) {
     free_plan(p);
     return fail(FCWT_HIP_EDEVICE, std::string(what) + ": hipStreamCreate failed");
   }
@@ -1564,6 +1605,79 @@ int fcwt_hip_cwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, 
 
 int fcwt_hip_ccwt_device(fcwt_hip_plan p, const float* d_input, float* d_output, void* stream) {
   return run_device(p, d_input, true, d_output, stream, "fcwt_hip_ccwt_device");
+}
+
+}  // extern "C"
+
+// ------------------------------------------------ no-allocation entry point
+// The device state of fcwt_hip_cwt_device_noalloc: one plan's tables,
+// spectrum, schedule and scratch, carved from this array (each GPU the
+// library runs on holds its own copy until the process ends).  In host code
+// the name is not a device address: the per-device address comes from
+// hipGetSymbolAddress, once per device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocSlot {
+  bool ready = false;      // ptr looked up on this device
+  char* ptr = nullptr;     // the device address of `scratch` on this device
+  fcwt_hip_plan plan = nullptr;  // the plan whose state the array holds
+  std::string key;         // what that plan was built for
+};
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+NoAllocSlot g_noalloc[kNoAllocDevices];
+std::atomic<unsigned> g_tune_generation{0};  // bumped when the tuning table changes
+}  // namespace
+
+extern "C" {
+
+int fcwt_hip_cwt_device_noalloc(const float* d_input, int n, const float* sc, int fn, float fb, int threads,
+                                int normalize, unsigned flags, float* d_output, void* stream) {
+  const char* what = "fcwt_hip_cwt_device_noalloc";
+  if (!d_input || !d_output || !sc) return fail(FCWT_HIP_EINVAL, std::string(what) + ": null argument");
+  if (n < 1 || fn < 1) return fail(FCWT_HIP_EINVAL, std::string(what) + ": need n >= 1 and fn >= 1");
+  if ((1LL << find2power(n)) > kNoAllocMaxN)
+    return fail(FCWT_HIP_EUNSUPPORTED, std::string(what) + ": n = " + std::to_string(n) +
+                                           " exceeds the limit of " + std::to_string(kNoAllocMaxN) +
+                                           " samples (padded size 2^17) of the no-allocation entry point");
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(FCWT_HIP_EUNSUPPORTED, std::string(what) + ": device index");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  NoAllocSlot& slot = g_noalloc[dev];
+  if (!slot.ready) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(FCWT_HIP_EDEVICE, std::string(what) + ": hipGetSymbolAddress: " + hipGetErrorString(e));
+    slot.ptr = (char*)ptr;
+    slot.ready = true;
+  }
+  // the plan the array holds is reused while the call asks for the same one
+  std::string key;
+  {
+    std::ostringstream ks;
+    ks << n << ' ' << fn << ' ' << fb << ' ' << threads << ' ' << (normalize != 0) << ' ' << flags << ' '
+       << g_tune_generation.load() << ' ';
+    key = ks.str();
+    key.append(reinterpret_cast<const char*>(sc), sizeof(float) * (size_t)fn);
+  }
+  if (!slot.plan || slot.key != key) {
+    if (slot.plan) {
+      free_plan(slot.plan);
+      slot.plan = nullptr;
+      slot.key.clear();
+    }
+    fcwt_hip_plan p = nullptr;
+    int rc = create_plan(&p, 1, &n, &sc, &fn, fb, threads, normalize, flags, what, slot.ptr, kNoAllocBytes,
+                         (hipStream_t)stream);
+    if (rc != FCWT_HIP_OK) return rc;
+    slot.plan = p;
+    slot.key = key;
+  }
+  float2* o = (float2*)d_output;
+  return launch_all(slot.plan, &d_input, false, &o, (hipStream_t)stream);
 }
 
 int fcwt_hip_spectrum_device(fcwt_hip_plan p, const float* d_input, float* d_spec, int complex_input, void* stream) {
@@ -1986,6 +2100,7 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = best;
+  ++g_tune_generation;
   return FCWT_HIP_OK;
 }
 
@@ -2014,6 +2129,7 @@ int fcwt_hip_tune_load(const char* path) {
     return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: bad contents: ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = t;
+  ++g_tune_generation;
   return FCWT_HIP_OK;
 }
 

@@ -158,6 +158,21 @@ int fcwt_hip_cwt(fcwt_hip_plan plan, const float* input, float* output);
  * the default stream).  Returns once enqueued.  d_output holds fn*n complex64. */
 int fcwt_hip_cwt_device(fcwt_hip_plan plan, const float* d_input, float* d_output, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): fcwt_hip_cwt_device without a plan object, allocating
+ * nothing.  The plan's tables, spectrum, schedule and scratch live in a
+ * static device array of 64 MiB that the library holds on each GPU it runs
+ * on until the process ends; the arguments are those of fcwt_hip_plan_create
+ * followed by those of fcwt_hip_cwt_device.  Limits: n <= 131072 samples
+ * (padded size 2^17), and the plan must fit the array (FCWT_HIP_EUNSUPPORTED
+ * otherwise; 3000 scales at n = 100,000 need about 52 MiB).  The array keeps
+ * the last call's plan: a call with the same n, scales and options reuses
+ * it, another rebuilds it (host work, table kernels and one stream
+ * synchronisation).  Calls that overlap on one GPU race on the array. */
+int fcwt_hip_cwt_device_noalloc(const float* d_input, int n, const float* scales, int fn, float fb, int threads,
+                                int normalize, unsigned flags, float* d_output, void* stream);
+
 /* FCWT::cwt(complex<float>* in, int n, complex<float>* out, Scales*),
  * src/fcwt/fcwt.cpp:536-538 (core 447-529; complex branch 486-490: the
  * zero-padded complex signal goes through a forward c2c FFT instead of the
