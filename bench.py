@@ -7,20 +7,25 @@ output 3000 x 100,000 complex64 (2.4 GB) written to HBM.
 
 A step is one full cwt (forward FFT + every scale) on device pointers: the
 input is already in HBM and the output buffer preallocated there.  The value
-is output GB/s = 8 * fn * n bytes / step time (HIP events around the K steps).
+is output GB/s = 8 * fn * n bytes / step time (HIP events around the K steps,
+no per-kernel events inside them).
 
 The roofline entry is for the dominant kernel (the one with the most time):
 its algorithmic bytes per launch (8 B per complex64 output value it writes,
 i.e. 8 * n * rows it owns; 4 n + 4 N for the forward FFT) over its average
-launch time from HIP events recorded around each launch on the launch stream,
-against MI355X's 8 TB/s HBM peak.  `traffic` is the HBM bytes per launch of
+launch time from HIP events recorded around each launch on the launch stream
+(in a separate pass of up to 10 steps after the timed ones), against
+MI355X's 8 TB/s HBM peak.  `traffic` is the HBM bytes per launch of
 that kernel from rocprofv3 FETCH_SIZE and WRITE_SIZE counter runs, each scaled
 by what the counter reads for a device copy of known size; it is read from
 the summary that `--summarize-profiles` wrote under profiles/ (null if absent).
 
-The CPU baseline is the oracle's float32 port (numpy float32 FFTs plus the
-reference's window multiply in C) on one host core over the first
-`--cpu-rows` scales of the same workload (all 3000 by default).
+The CPU baseline is the oracle's float32 port (the reference's window
+multiply in C, then scipy float32 inverse FFTs) on the host's threads (a power
+of two, at most 16: one GPU's share of its host) over the first `--cpu-rows`
+scales of the same workload (all 3000 by default).  The reference's own
+FFTW+OpenMP build cannot be made here (FFTW exists only as prebuilt binaries
+inside the reference).
 
 Other workloads (each prints its own JSON line; not the headline metric):
   --workload long      n = 2^22 AR(2) + 3-segment tone, Morlet sigma 4, 1024
@@ -268,14 +273,38 @@ def workload_host(args):
                        "pipelined_vs_plain_speedup": round(res["plain_copy"] / res["pipelined"], 3)}}
 
 
-def cpu_baseline(x, scales, fb, rows):
-    """Oracle float32 port on one core over the first `rows` scales."""
+def host_threads(cap=16):
+    """Host threads for the CPU baseline: the largest power of two <= the
+    CPUs this process may run on, at most `cap` (a GPU's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    n = max(1, min(cap, n))
+    return 1 << (n.bit_length() - 1)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(x, scales, fb, rows, threads):
+    """Oracle float32 port (the reference's window multiply with its O1
+    reuse, then a full N-point float32 inverse FFT per scale) over the first
+    `rows` scales, on `threads` host threads."""
     from oracle import oracle as O
 
     out = np.empty((rows, x.size), np.complex64)
-    O.cwt_fp32_port(x, scales[:rows], fb, rows=rows, out=out)  # warm (imports, tables)
+    O.cwt_fp32_port(x, scales[:min(rows, 64)], fb, rows=min(rows, 64), out=out, workers=threads)  # warm
     t0 = time.perf_counter()
-    O.cwt_fp32_port(x, scales[:rows], fb, rows=rows, out=out)
+    O.cwt_fp32_port(x, scales[:rows], fb, rows=rows, out=out, workers=threads)
     dt = time.perf_counter() - t0
     return 8.0 * rows * x.size / dt / 1e9, dt
 
@@ -385,6 +414,122 @@ def summarize_profiles(root):
             summarize_trace(root, *spec)
     if os.path.exists(os.path.join(root, "pmc_fetch", "run_counter_collection.csv")):
         summarize_pmc(root)
+    if os.path.exists(os.path.join(root, "sq_a", "run_counter_collection.csv")):
+        summarize_sq(root)
+
+
+# instruction-mix counter passes (one rocprofv3 --pmc run each; at most 8 SQ counters per run)
+SQ_PASSES = {
+    "sq_a": ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR",
+             "SQ_INSTS_SMEM", "SQ_WAVES", "SQ_INSTS_BRANCH"],
+    "sq_b": ["SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_SCA", "SQ_WAIT_INST_ANY",
+             "SQ_WAVE_CYCLES", "SQ_BUSY_CU_CYCLES", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"],
+}
+# kernels summarized, with the rows they write in the 100k x 3000 workload
+SQ_KERNELS = [("k_rows_team", "team"), ("k_rows_direct<9>", "direct"), ("k_rows_direct<10>", "direct")]
+
+
+def headline_rows_per_kernel():
+    """Rows of the 100k x 3000 plan per row kernel, from the library's host
+    planning (fcwt_hip_describe_rows; no device): M = 2^ceil(log2 band) rows
+    of M <= 512 and 1024 go to the direct kernel's instances, wider ones to
+    the team kernel."""
+    import ctypes
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from fcwt_amd import _native as nat
+    import fcwt_amd
+
+    fn = 3000
+    sc = fcwt_amd.Scales(fcwt_amd.Morlet(2.0), fcwt_amd.FCWT_LOGSCALES, 1000, 1.0, 450.0, fn).scales
+    band = np.zeros(fn, np.int32)
+    nat.check(nat.lib().fcwt_hip_describe_rows(100_000, nat.fptr(sc), fn, 2.0, 1, 0, None,
+                                                band.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None),
+              "describe_rows")
+    log2M = np.ceil(np.log2(np.maximum(band, 1))).astype(int)
+    return {"k_rows_team": int((log2M > 10).sum()), "k_rows_direct<10>": int((log2M == 10).sum()),
+            "k_rows_direct<9>": int((log2M <= 9).sum())}
+
+
+def summarize_sq(root):
+    """profiles/row_kernels_100k_x_3000_instruction_mix.{json,md} from the
+    sq_a/ and sq_b/ counter runs (bench.py --steps 2 --warmup 1 --no-cpu) and
+    the trace/ run's average kernel durations: per dispatch counts, and per
+    point of the padded row (rows x N points) the VALU instructions of one lane."""
+    import csv
+
+    N, n = 131072, 100_000
+    rows_of = headline_rows_per_kernel()
+    dur = {}
+    tpath = os.path.join(root, "trace", "run_kernel_stats.csv")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            for r in csv.DictReader(f):
+                dur[r["Name"]] = float(r["AverageNs"])
+    per = {}
+    for d, ctrs in SQ_PASSES.items():
+        for kname, _ in SQ_KERNELS:
+            for c in ctrs:
+                try:
+                    v, nd = _pmc_per_dispatch(os.path.join(root, d, "run_counter_collection.csv"), c, (kname,))
+                except SystemExit:
+                    continue
+                per.setdefault(kname, {})[c] = v
+    out = {"workload": WORKLOAD_TAG, "N": N, "n": n, "kernels": {}}
+    cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {c} --output-format csv "
+           "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py --steps 2 "
+           "--warmup 1 --no-cpu")
+    out["commands"] = [cmd.format(c=" ".join(v), d=k) for k, v in SQ_PASSES.items()]
+    for kname, kind in SQ_KERNELS:
+        if kname not in per:
+            continue
+        c = per[kname]
+        rows = (rows_of or {}).get(kname)
+        ent = {"per_dispatch": c, "rows": rows}
+        ns = next((v for k, v in dur.items() if kname.replace("<", "<") in k), None)
+        ent["avg_ns"] = ns
+        if rows:
+            pts = rows * N
+            ent["valu_per_point"] = c.get("SQ_INSTS_VALU", 0) * 64 / pts
+            ent["salu_per_point"] = c.get("SQ_INSTS_SALU", 0) * 64 / pts
+            ent["lds_per_point"] = c.get("SQ_INSTS_LDS", 0) * 64 / pts
+        if ns and "SQ_INSTS_VALU" in c:
+            # one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz
+            ent["valu_floor_ns"] = c["SQ_INSTS_VALU"] / (256 * 2.4)
+            ent["valu_issue_fraction"] = ent["valu_floor_ns"] / ns
+        if c.get("SQ_LDS_IDX_ACTIVE"):
+            ent["lds_bank_conflict_fraction"] = c.get("SQ_LDS_BANK_CONFLICT", 0) / c["SQ_LDS_IDX_ACTIVE"]
+        out["kernels"][kname] = ent
+    base = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"row_kernels_{WORKLOAD_TAG}_instruction_mix")
+    with open(base + ".json", "w") as f:
+        json.dump(out, f, indent=1)
+    with open(base + ".md", "w") as f:
+        f.write("# Instruction mix of the row kernels, cwt 100k x 3000 (rocprofv3 --pmc)\n\n")
+        f.write("Commands (on the MI355X, one counter set per run):\n\n")
+        for c in out["commands"]:
+            f.write(f"    {c}\n")
+        f.write("\nSummarized by `python bench.py --summarize-profiles gpurun_out/prof` (average kernel durations "
+                "from the trace/ run of `profiles/cwt_100k_x_3000_kernel_trace.md`). Counts are per dispatch, "
+                "summed over the device; per point = x 64 lanes / (rows x N), N = 131072. The VALU floor is "
+                "SQ_INSTS_VALU / (256 CUs x 2.4 GHz), one wave64 VALU instruction per CU per clock.\n\n")
+        keys = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_SMEM", "SQ_INSTS_VMEM_RD",
+                "SQ_INSTS_VMEM_WR", "SQ_INSTS_BRANCH", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU",
+                "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"]
+        f.write("| kernel | rows | avg us | " + " | ".join(keys) + " | VALU/pt | SALU/pt | LDS/pt | VALU floor us "
+                "| VALU issue | LDS conflict |\n")
+        f.write("|" + "---|" * (len(keys) + 9) + "\n")
+        for kname, e in out["kernels"].items():
+            c = e["per_dispatch"]
+            cells = [f"{c[k] / 1e6:.2f} M" if k in c else "-" for k in keys]
+            def g(k, fmt, scale=1.0):
+                return fmt.format(e[k] * scale) if e.get(k) is not None else "-"
+
+            f.write(f"| `{kname}` | {e['rows'] or '-'} | {g('avg_ns', '{:.1f}', 1e-3)} | " + " | ".join(cells)
+                    + f" | {g('valu_per_point', '{:.1f}')} | {g('salu_per_point', '{:.1f}')}"
+                    + f" | {g('lds_per_point', '{:.1f}')} | {g('valu_floor_ns', '{:.1f}', 1e-3)}"
+                    + f" | {g('valu_issue_fraction', '{:.2f}')} | {g('lds_bank_conflict_fraction', '{:.3f}')} |\n")
+    print(json.dumps(out)[:2000])
 
 
 def summarize_pmc(root):
@@ -498,7 +643,6 @@ def main():
         step()
     torch.cuda.synchronize()
     plan.check()
-    plan.profile(True)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -508,6 +652,13 @@ def main():
     e1.record(stream)
     torch.cuda.synchronize()
     ms_step = e0.elapsed_time(e1) / args.steps
+    plan.check()
+    # per-kernel times in a separate pass of the same steps, with HIP events
+    # around every launch on the launch stream (kept out of the timed steps)
+    prof_steps = max(1, min(args.steps, 10))
+    plan.profile(True)
+    for _ in range(prof_steps):
+        step()
     prof = plan.profile_read()
     plan.profile(False)
     plan.check()
@@ -517,7 +668,7 @@ def main():
     kernels = {}
     for k, (ms, c) in prof.items():
         b = kind_bytes(k, n, plan.N, split)
-        kernels[k] = {"ms_per_launch": ms / c, "launches_per_step": c / args.steps,
+        kernels[k] = {"ms_per_launch": ms / c, "launches_per_step": c / prof_steps,
                       "bytes_per_launch": b, "gbs": b / (ms / c * 1e-3) / 1e9}
     dom = max(prof, key=lambda k: prof[k][0])
     d = kernels[dom]
@@ -552,11 +703,13 @@ def main():
     }
     if not args.no_cpu:
         rows = min(args.cpu_rows, fn)
-        gbs, dt = cpu_baseline(x, scs.scales, fb, rows)
-        result["cpu_baseline"] = {"value": round(gbs, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-                                  "sample": f"first {rows} of {fn} scales of the same workload "
-                                            f"({dt:.1f} s; oracle float32 port: numpy float32 FFTs + reference "
-                                            f"window multiply in C)"}
+        threads = host_threads()
+        gbs, dt = cpu_baseline(x, scs.scales, fb, rows, threads)
+        result["cpu_baseline"] = {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                                  "sample": f"first {rows} of {fn} scales of the same workload ({dt:.2f} s on "
+                                            f"{threads} threads of {cpu_model()}; oracle float32 port: the "
+                                            f"reference's window multiply in C, then a full {plan.N}-point float32 "
+                                            f"inverse FFT per scale (scipy.fft), blocks of 32 scales per thread)"}
     print(json.dumps(result))
 
 

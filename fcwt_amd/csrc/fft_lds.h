@@ -32,6 +32,20 @@ __device__ __forceinline__ void gstore_nt(gfloat2* p, float2 v) {
                               reinterpret_cast<gu64*>(p));
 }
 
+// Stores of output rows through a buffer resource: the hardware range check
+// drops a store at an element offset >= the row length, so a partial run of
+// a row is written without a branch or an exec mask per store.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// the resource for `count` complex64 values at p (count * 8 < 2^32)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(float2* p, long long count) {
+  return __builtin_amdgcn_make_buffer_rsrc(p, 0, (int)(unsigned)(count * 8), 0x00020000);
+}
+// row[idx] = v with the non-temporal hint (written once, never re-read)
+__device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned idx, float2 v) {
+  const u32x2 u = {__float_as_uint(v.x), __float_as_uint(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(u, r, idx * 8u, 0, 2);
+}
+
 constexpr int kLdsCap = 16384;  // complex64 elements per work-group (128 KiB)
 constexpr int kBlock = 1024;    // threads per LDS-FFT work-group
 

@@ -324,13 +324,19 @@ __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, in
 #pragma unroll
   for (int k2 = 0; k2 < 16; ++k2) xb[(k1 + 16 * k2) * 17 + g] = v[k2];
   __syncthreads();
-  gfloat2* out = (gfloat2*)row_out(a.k, r);
+  // every LDS read first, then the stores, clipped at t >= n by the row's
+  // buffer range
+  float2 o[16];
 #pragma unroll
   for (int s = 0; s < 16; ++s) {
     const int e = s * 256 + tid;
-    const int m = e >> 4, gg = e & 15;
-    const long long t = (long long)(f0 + gg) + ((long long)m << LOG2F);
-    if (t < r.n) gstore_nt(out + t, xb[m * 17 + gg]);
+    o[s] = xb[(e >> 4) * 17 + (e & 15)];
+  }
+  const __amdgpu_buffer_rsrc_t rr = row_rsrc(row_out(a.k, r), r.n);
+#pragma unroll
+  for (int s = 0; s < 16; ++s) {
+    const int e = s * 256 + tid;
+    bstore_nt(rr, (unsigned)(f0 + (e & 15)) + ((unsigned)(e >> 4) << LOG2F), o[s]);
   }
 }
 

@@ -168,10 +168,28 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
           for (int k2 = 0; k2 < KH; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + h * KH + k2];
       }
       __syncthreads();
-      for (int e = tid; e < MH * G; e += NT) {
-        const int mm = e / G, gg = e - mm * G;
-        const long long t = (long long)(l0 + gg) + ((long long)(mm + h * MH) << log2L);
-        if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[mm * C::OS + gg]);
+      if (G == C::G) {
+        // every LDS read first, then the stores, clipped at t >= n by the
+        // row's buffer range
+        constexpr int S = MH * C::G / NT;
+        float2 o[S];
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int e = s * NT + tid;
+          o[s] = buf[(e / C::G) * C::OS + e % C::G];
+        }
+        const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          const int e = s * NT + tid;
+          bstore_nt(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
+        }
+      } else {
+        for (int e = tid; e < MH * G; e += NT) {
+          const int mm = e / G, gg = e - mm * G;
+          const long long t = (long long)(l0 + gg) + ((long long)(mm + h * MH) << log2L);
+          if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[mm * C::OS + gg]);
+        }
       }
       if (h == 0) __syncthreads();
     }

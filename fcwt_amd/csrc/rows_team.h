@@ -362,9 +362,18 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
       }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < kTeamPoints; e += kTeamThreads) {
-    const int f = e >> log2Qb, ql = e & ((1 << log2Qb) - 1);
-    gstore(U + f * kTeamB + qb0 + ql, buf[f * TS + ql]);
+  // every LDS read first, then the stores
+  constexpr int S = kTeamPoints / kTeamThreads;
+  float2 o[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int e = s * kTeamThreads + threadIdx.x;
+    o[s] = buf[(e >> log2Qb) * TS + (e & ((1 << log2Qb) - 1))];
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int e = s * kTeamThreads + threadIdx.x;
+    gstore(U + (e >> log2Qb) * kTeamB + qb0 + (e & ((1 << log2Qb) - 1)), o[s]);
   }
 }
 
@@ -409,12 +418,20 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
 #pragma unroll
   for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = w[k2];
   __syncthreads();
+  // every LDS read first, then the stores, which the row's buffer range
+  // clips at t >= n
+  constexpr int S = (kTeamB * 16) / kTeamThreads;
+  float2 o[S];
 #pragma unroll
-  for (int s = 0; s < (kTeamB * 16) / kTeamThreads; ++s) {
+  for (int s = 0; s < S; ++s) {
     const int e = s * kTeamThreads + threadIdx.x;
-    const int m = e >> 4, gg = e & 15;
-    const long long t = (long long)(f0 + gg) + ((long long)m << LOG2F);
-    if (t < n) gstore_nt(out + t, buf[m * OS + gg]);
+    o[s] = buf[(e >> 4) * OS + (e & 15)];
+  }
+  const __amdgpu_buffer_rsrc_t rows = row_rsrc((float2*)out, n);
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const int e = s * kTeamThreads + threadIdx.x;
+    bstore_nt(rows, (unsigned)(f0 + (e & 15) + ((e >> 4) << LOG2F)), o[s]);
   }
 }
 

@@ -214,8 +214,15 @@ def cwt_fp64(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bo
 
 
 def cwt_fp32_port(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, avx: bool = True,
-                  normalize: bool = True, rows=None, out=None) -> np.ndarray:
-    """The same path in float32 with numpy's float32 FFTs (CPU baseline port)."""
+                  normalize: bool = True, rows=None, out=None, workers: int = 1, block: int = 32) -> np.ndarray:
+    """The same path in float32 (CPU baseline port): the reference's window
+    multiply with its O1 reuse (C, fcwt.cpp:197-311) row after row, and the
+    full N-point unnormalised inverse FFT of each row (fcwt.cpp:418-429) with
+    scipy's float32 FFT, over blocks of ``block`` rows on ``workers`` host
+    threads (the reference runs its scale loop on T OpenMP/FFTW threads,
+    fcwt.cpp:217-220, 469-477).  The result does not depend on ``workers``."""
+    import scipy.fft as sfft
+
     x = _signal(x)
     n = x.size
     N = 1 << find2power(n)
@@ -227,13 +234,29 @@ def cwt_fp32_port(x: np.ndarray, sc: np.ndarray, fb: float, threads: int = 1, av
     nrow = sc.size if rows is None else int(rows)
     if out is None:
         out = np.zeros((nrow, n), np.complex64)
-    O1 = np.zeros(N, np.complex64)
-    for i in range(nrow):
-        window_multiply(X, O1[: N // 2], mom, float(sc[i]), N, threads, avx)
-        y = np.fft.ifft(O1, norm="forward")  # unnormalised backward DFT
+    from concurrent.futures import ThreadPoolExecutor
+
+    def finish(buf, r0, cnt):  # numpy and scipy release the GIL here
+        y = sfft.ifft(buf, axis=1, norm="forward", overwrite_x=True)  # unnormalised backward DFT
         if normalize:
-            y = y * np.float32(1.0 / N)
-        out[i] = y[:n]
+            y *= np.float32(1.0 / N)
+        out[r0: r0 + cnt] = y[:, :n]
+
+    O1 = np.zeros(N // 2, np.complex64)
+    block = max(1, min(block, nrow))
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as pool:
+        jobs = []
+        for r0 in range(0, nrow, block):
+            cnt = min(block, nrow - r0)
+            buf = np.zeros((cnt, N), np.complex64)
+            for k in range(cnt):  # the O1 reuse makes this part sequential
+                window_multiply(X, O1, mom, float(sc[r0 + k]), N, threads, avx)
+                buf[k, : N // 2] = O1
+            jobs.append(pool.submit(finish, buf, r0, cnt))
+            if len(jobs) > 2 * workers:
+                jobs.pop(0).result()
+        for j in jobs:
+            j.result()
     return out
 
 

@@ -287,6 +287,10 @@ def test_cwt_fp32_port_matches_fp64():
     a = O.cwt_fp32_port(x, sc, 2.0)
     b = O.cwt_fp64(x, sc, 2.0)
     assert O.rel_l2_rows(a, b).max() < 1e-5
+    # blocks of rows on several threads give the same values (the O1 reuse
+    # crosses block edges: block = 7 splits the 30 rows unevenly)
+    c = O.cwt_fp32_port(x, sc, 2.0, workers=4, block=7)
+    assert np.array_equal(a, c)
 
 
 def _o1_composition_matches(case):
