@@ -136,17 +136,22 @@ __device__ __forceinline__ void team_fail(unsigned* abort, unsigned* error, unsi
 }
 
 // Thread 0 waits until *p >= target (at most `spin` polls, and not after the
-// launch has aborted); the result is broadcast.  spin == 0 fails every wait
-// at once: the host sets it only to test the failure path.
-__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, int* s_flag, unsigned* abort,
-                                          unsigned* error, long spin) {
+// launch has aborted); the result is broadcast.  `first` is a value of *p
+// thread 0 read earlier in the item (its load overlapped the item's own
+// loads); the counter only grows, so when it already meets the target no
+// further poll is made.  spin == 0 fails every wait at once: the host sets it
+// only to test the failure path.
+__device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, unsigned first, int* s_flag,
+                                          unsigned* abort, unsigned* error, long spin) {
   if (threadIdx.x == 0) {
     long s = 0;
-    while (ld_agent(p) < target && ld_agent(abort) == 0 && s < spin) {
+    unsigned v = first;
+    while (v < target && ld_agent(abort) == 0 && s < spin) {
       __builtin_amdgcn_s_sleep(1);
+      v = ld_agent(p);
       ++s;
     }
-    bool ok = spin > 0 && ld_agent(p) >= target;
+    bool ok = spin > 0 && v >= target;
     if (!ok) team_fail(abort, error, 1u);
     *s_flag = ok;
   }
@@ -525,6 +530,12 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     out += out_off;
     const int b = i % nbuf, gen = i / nbuf;
     gfloat2* const U = (gfloat2*)(scratch + (size_t)b * N);
+    // the counter this item waits on (pass 1: the buffer's pass-2 readers of
+    // the row nbuf earlier; pass 2: the row's pass-1 arrivals), read now so
+    // that the read overlaps the item's own loads
+    const unsigned* const dep = arrive + 64 * b + ((sq & 1) ? 0 : 32);
+    unsigned first = 0;
+    if (threadIdx.x == 0 && ((sq & 1) || gen > 0)) first = ld_agent(dep);
     if ((sq & 1) == 0) {
       // ---- pass 1 of row i, qb block blk
       const int qb0 = blk << log2Qb;
@@ -578,7 +589,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // the buffer's previous row has been read by every pass-2 item (the
       // wait ends in a barrier, which also publishes the staged band)
       if (gen > 0) {
-        if (!team_wait(arrive + 64 * b + 32, (unsigned)(gen << log2I), &s_flag, abort, error, spin)) break;
+        if (!team_wait(dep, (unsigned)(gen << log2I), first, &s_flag, abort, error, spin)) break;
       } else {
         __syncthreads();
       }
@@ -597,7 +608,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // W_N^{j f} times the row normalisation
       const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
-      if (!team_wait(arrive + 64 * b, (unsigned)((gen + 1) << log2I), &s_flag, abort, error, spin)) break;
+      if (!team_wait(dep, (unsigned)((gen + 1) << log2I), first, &s_flag, abort, error, spin)) break;
       team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
     }
     // publishes the next item, and every thread is done with this item's LDS
