@@ -277,15 +277,45 @@ template <int LOG2H, bool DERIVED>
 __global__ __launch_bounds__(kRowFftThreads) void k_rows_fft(FwdArgs a) {
   __shared__ RowFftLds<LOG2H> s;
   constexpr int H = 1 << LOG2H;
+  constexpr int PER = (H + kRowFftThreads - 1) / kRowFftThreads;  // points per thread
   const float* x = fwd_input(a, blockIdx.y);
   const int nx = fwd_len(a, blockIdx.y);
   float2* spec = a.spec + blockIdx.y * a.spec_stride;
   row_tw_tables<LOG2H>(s);
-  for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[row_pad(j)] = packed(x, nx, j);
+  if ((nx & 1) == 0) {
+    // whole complex values: every load issued before the first is used;
+    // the buffer range (nx floats) reads zero past the input, which is the
+    // zero padding
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void*)x, 0, nx * 4, 0x00020000);
+    float2 v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = threadIdx.x + u * kRowFftThreads;
+      const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rx, (unsigned)j * 8u, 0, 0);
+      v[u] = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = threadIdx.x + u * kRowFftThreads;
+      if (j < H) s.data[row_pad(j)] = v[u];
+    }
+  } else {
+    for (int j = threadIdx.x; j < H; j += kRowFftThreads) s.data[row_pad(j)] = packed(x, nx, j);
+  }
   __syncthreads();
   row_fft<LOG2H, -1, DERIVED>(s);
   if (a.complex_in) {
-    for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[row_pad(k)];
+    if (a.keep == H / 2 && H >= 2 * kRowFftThreads) {
+      // every LDS read first, then the stores
+      constexpr int KP = H / 2 / kRowFftThreads;
+      float2 o[KP > 0 ? KP : 1];
+#pragma unroll
+      for (int u = 0; u < KP; ++u) o[u] = s.data[row_pad(threadIdx.x + u * kRowFftThreads)];
+#pragma unroll
+      for (int u = 0; u < KP; ++u) spec[threadIdx.x + u * kRowFftThreads] = o[u];
+    } else {
+      for (int k = threadIdx.x; k < a.keep; k += kRowFftThreads) spec[k] = s.data[row_pad(k)];
+    }
     return;
   }
   for (int k = threadIdx.x; k < H; k += kRowFftThreads) {
