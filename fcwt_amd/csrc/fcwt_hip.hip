@@ -482,6 +482,90 @@ __global__ void k_adj_reduce(const float2* P, long long row0, int count, int fn,
   a[q] = hi == s0 + fn ? make_float2(acc.x, -acc.y) : acc;
 }
 
+// Rows of the fused adjoint per work-group: a work-group transforms this many
+// consecutive gradient rows of one signal and sums their windowed spectra in
+// registers.  Fixed, so that a signal's sum is grouped the same way in a
+// batch as alone.
+constexpr int kAdjRows = 4;
+
+// The adjoint's row transforms fused with the window and the sum over rows,
+// for N = 2^LOG2H <= 2^14 (one work-group holds a row's transform in LDS).
+// Work-group (g, s) takes rows [g R, g R + R) of signal s (R = kAdjRows, fn
+// rows per signal, n complex values each at grad + (s fn + i) n):
+//   Ghat_i = FFT_N(G_i) (zero-padded), acc[q] += w_i(q) Ghat_i[q] for q < N/2,
+// where the window is nonzero (the row's band), and stores acc to
+// part[(s groups + g) N/2 + q].  Each gradient row is read once and its
+// spectrum never leaves the chip.
+template <int LOG2H>
+__global__ __launch_bounds__(kRowFftThreads) void k_adj_rows(const float2* grad, int n, int fn, int groups,
+                                                            const RowD* rows, const SegD* segs,
+                                                            const float* mother, float2* part) {
+  __shared__ RowFftLds<LOG2H> s;
+  constexpr int H = 1 << LOG2H, HALF = H / 2;
+  constexpr int PER = (H + kRowFftThreads - 1) / kRowFftThreads;
+  constexpr int KQ = (HALF + kRowFftThreads - 1) / kRowFftThreads;
+  const int g = blockIdx.x, sig = blockIdx.y;
+  const int i0 = g * kAdjRows, i1 = min(fn, i0 + kAdjRows);
+  const float maxidx = (float)H - 1.0f;
+  float2 acc[KQ];
+#pragma unroll
+  for (int u = 0; u < KQ; ++u) acc[u] = make_float2(0.f, 0.f);
+  row_tw_tables<LOG2H>(s);
+  for (int i = i0; i < i1; ++i) {
+    const float2* x = grad + ((size_t)sig * fn + i) * (size_t)n;
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void*)x, 0, n * 8, 0x00020000);
+    float2 v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(rx, (unsigned)(threadIdx.x + u * kRowFftThreads) * 8u, 0, 0);
+      v[u] = make_float2(__uint_as_float(w.x), __uint_as_float(w.y));
+    }
+    __syncthreads();  // the previous row's spectrum has been read
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int j = threadIdx.x + u * kRowFftThreads;
+      if (j < H) s.data[row_pad(j)] = v[u];
+    }
+    __syncthreads();
+    row_fft<LOG2H, -1, true>(s);
+    // the row's band: the last bin any of its segments covers
+    const RowD r = rows[i];
+    int qend = 0;
+    for (int k = 0; k < r.seg_count; ++k) qend = max(qend, segs[r.seg_begin + k].hi);
+#pragma unroll
+    for (int u = 0; u < KQ; ++u) {
+      const int q = threadIdx.x + u * kRowFftThreads;
+      if (q < HALF && q < qend) {
+        const float w = row_window(segs, r, q, maxidx, mother);
+        const float2 z = s.data[row_pad(q)];
+        acc[u].x += w * z.x;
+        acc[u].y += w * z.y;
+      }
+    }
+  }
+  float2* out = part + ((size_t)sig * groups + g) * HALF;
+#pragma unroll
+  for (int u = 0; u < KQ; ++u) {
+    const int q = threadIdx.x + u * kRowFftThreads;
+    if (q < HALF) out[q] = acc[u];
+  }
+}
+
+// A_s[q] = conj(sum over the groups g of part[(s groups + g) N/2 + q]), in
+// group order, for q < N/2; s = blockIdx.y.
+__global__ void k_adj_combine(const float2* part, int groups, int half, float2* A) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= half) return;
+  const float2* p = part + (size_t)blockIdx.y * groups * half + q;
+  float2 a = make_float2(0.f, 0.f);
+  for (int g = 0; g < groups; ++g) {
+    const float2 v = p[(size_t)g * half];
+    a.x += v.x;
+    a.y += v.y;
+  }
+  A[(size_t)blockIdx.y * half + q] = make_float2(a.x, -a.y);
+}
+
 // xbar_s[t] = c Re y_s[t] (real input) or c conj(y_s[t]) (complex input),
 // t < n, s = blockIdx.y, where y_s = FFT_N(conj A_s) (N apart).
 __global__ void k_adj_out(const float2* y, int n, int N, float c, int complex_in, float* xbar) {
@@ -1073,10 +1157,38 @@ static int adjoint_workspace(fcwt_hip_plan p, int signals) {
   return FCWT_HIP_OK;
 }
 
+template <int LOG2H>
+static void launch_adj_rows_t(dim3 grid, const float2* g, int n, int fn, int groups, fcwt_hip_plan p, float2* part,
+                              hipStream_t st) {
+  k_adj_rows<LOG2H><<<grid, kRowFftThreads, 0, st>>>(g, n, fn, groups, p->d_rows, p->d_segs, p->d_mother, part);
+}
+
+static void launch_adj_rows(int log2H, dim3 grid, const float2* g, int n, int fn, int groups, fcwt_hip_plan p,
+                            float2* part, hipStream_t st) {
+  switch (log2H) {
+    case 1: launch_adj_rows_t<1>(grid, g, n, fn, groups, p, part, st); break;
+    case 2: launch_adj_rows_t<2>(grid, g, n, fn, groups, p, part, st); break;
+    case 3: launch_adj_rows_t<3>(grid, g, n, fn, groups, p, part, st); break;
+    case 4: launch_adj_rows_t<4>(grid, g, n, fn, groups, p, part, st); break;
+    case 5: launch_adj_rows_t<5>(grid, g, n, fn, groups, p, part, st); break;
+    case 6: launch_adj_rows_t<6>(grid, g, n, fn, groups, p, part, st); break;
+    case 7: launch_adj_rows_t<7>(grid, g, n, fn, groups, p, part, st); break;
+    case 8: launch_adj_rows_t<8>(grid, g, n, fn, groups, p, part, st); break;
+    case 9: launch_adj_rows_t<9>(grid, g, n, fn, groups, p, part, st); break;
+    case 10: launch_adj_rows_t<10>(grid, g, n, fn, groups, p, part, st); break;
+    case 11: launch_adj_rows_t<11>(grid, g, n, fn, groups, p, part, st); break;
+    case 12: launch_adj_rows_t<12>(grid, g, n, fn, groups, p, part, st); break;
+    case 13: launch_adj_rows_t<13>(grid, g, n, fn, groups, p, part, st); break;
+    default: launch_adj_rows_t<14>(grid, g, n, fn, groups, p, part, st); break;
+  }
+}
+
 // Input gradients of `signals` signals from their output gradients d_grad
-// (signals x fn rows of n complex64, contiguous): per chunk of rows,
-// Ghat_r = FFT_N(G_r)[0, N/2) and A_s += w_i Ghat_r; then xbar_s =
-// c Re IFFT_N(A_s)[0, n) (the complex value for complex input), n apart.
+// (signals x fn rows of n complex64, contiguous): Ghat_r = FFT_N(G_r)[0, N/2)
+// and A_s = sum over s's rows of w_i Ghat_r (at N <= 2^14 fused in
+// k_adj_rows, beyond that per chunk of rows through the workspace); then
+// xbar_s = c Re IFFT_N(A_s)[0, n) (the complex value for complex input), n
+// apart.
 static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, int signals, bool complex_in, float* d_xbar,
                           hipStream_t st) {
   const int n = p->n, N = p->N, fn = p->fn;
@@ -1104,7 +1216,23 @@ static int launch_adjoint(fcwt_hip_plan p, const float* d_grad, int signals, boo
   f.tw = twiddles_of(p);
   ProfScope ps(p, 5, st);
   const long long rows = (long long)fn * signals;
-  for (long long row0 = 0; row0 < rows; row0 += p->adj_chunk) {
+  const int groups = (fn + kAdjRows - 1) / kAdjRows;
+  const bool fused = p->log2N <= 14 && groups <= p->adj_chunk;
+  if (fused) {
+    // fused: row transforms, windows and the sum over rows in one kernel,
+    // partial sums per group of rows in the workspace, signals in chunks
+    const int per = std::max(1, std::min(p->adj_chunk / groups, 65535));
+    for (int s0 = 0; s0 < signals; s0 += per) {
+      const int cnt = std::min(per, signals - s0);
+      const float2* g0 = (const float2*)d_grad + (size_t)s0 * fn * n;
+      launch_adj_rows(p->log2N, dim3(groups, cnt), g0, n, fn, groups, p, p->d_adj_P, st);
+      k_adj_combine<<<dim3((N / 2 + 255) / 256, cnt), 256, 0, st>>>(p->d_adj_P, groups, N / 2,
+                                                                    p->d_adj_A + (size_t)s0 * (N / 2));
+    }
+  }
+  // otherwise, per chunk of rows: their spectra into the workspace, then
+  // the windowed sum over rows
+  for (long long row0 = 0; row0 < rows && !fused; row0 += p->adj_chunk) {
     const int count = (int)std::min<long long>(p->adj_chunk, rows - row0);
     f.x = d_grad + (size_t)row0 * 2 * n;
     f.spec = p->d_adj_P;
