@@ -596,6 +596,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-rows", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the tuned-schedule comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--summarize-profiles", default="", metavar="DIR")
     ap.add_argument("--workload", default="headline", choices=["headline", "long", "batched", "adjoint", "host"])
@@ -663,6 +664,39 @@ def main():
     plan.profile(False)
     plan.check()
 
+    # the same workload after create_FFT_optimization_plan's analogue: the
+    # schedule of N tuned on this GPU (fcwt_hip_tune, FFTW_PATIENT effort),
+    # loaded, and a new plan timed the same way (reported beside the default)
+    tuned = None
+    if not args.no_tune:
+        import tempfile
+
+        from fcwt_amd import _native as nat
+
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, f"n{plan.N}_t1.fcwt_hip").encode()
+            t0 = time.perf_counter()
+            nat.check(nat.lib().fcwt_hip_tune(plan.N, 1, 2, path), "fcwt_hip_tune")
+            tune_s = time.perf_counter() - t0
+            with open(path) as f:
+                chosen = dict(line.split(None, 1) for line in f.read().strip().splitlines()[1:])
+            nat.check(nat.lib().fcwt_hip_tune_load(path), "fcwt_hip_tune_load")
+        tplan = fcwt_amd.Plan(n, scs.scales, fb, threads=1, normalize=True)
+        for _ in range(args.warmup):
+            tplan.execute_device(dx.data_ptr(), out.data_ptr(), stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(args.steps):
+            tplan.execute_device(dx.data_ptr(), out.data_ptr(), stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        tplan.check()
+        tms = e0.elapsed_time(e1) / args.steps
+        tuned = {"ms_per_step": round(tms, 4), "vs_default": round(ms_step / tms, 4), "tune_s": round(tune_s, 2),
+                 "schedule": {k: chosen[k].strip() for k in ("team_buffers", "team_lag", "direct_max", "path")
+                              if k in chosen}}
+        tplan.close()
+
     out_bytes = 8.0 * fn * n
     value = out_bytes / (ms_step * 1e-3) / 1e9
     kernels = {}
@@ -691,7 +725,8 @@ def main():
                                "Morlet sigma=2, 3000 LOGSCALES 1-450 Hz, complex64 output 2.4 GB",
                    "n": n, "fn": fn, "N": plan.N, "rows": split,
                    "samples_scales_per_s": round(fn * n / (ms_step * 1e-3), 1),
-                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2)},
+                   "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
+                   "tuned_schedule": tuned},
         "roofline": {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(d["gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": round(tr["traffic_bytes_per_launch"]) if tr else None,

@@ -616,6 +616,9 @@ struct Tuned {
   int team_buffers = 0;  // XCD-team scratch buffers per team
   int cols_mb = 0;       // column path: scratch MB per batch of rows
   int path = 0;          // wide rows: 0 built-in, 1 XCD-team kernel, 2 column path
+  int team_lag = 0;      // steps between a row's pass 1 and its pass 2 in a team's sequence (< team_buffers)
+  int direct_max = 0;    // widest band (log2 M) on the single-work-group kernel: 9 or 10
+  int cols_sets = 0;     // column path with several batches: 1 separate passes, 2 fused launches
 };
 static std::mutex g_tune_mu;
 static std::map<int, Tuned> g_tuned;
@@ -661,7 +664,7 @@ struct fcwt_hip_plan_s {
   std::vector<Item> items;  // direct writer items, then per batch: spreader items, writer items
   // XCD-team path for wide bands (rows_team.h)
   bool team_capable = false;
-  int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, seq_stride = 0;
+  int team_size = 0, nteams = 0, team_log2F = 0, team_nbuf = 2, team_lag = 1, seq_stride = 0;
   std::vector<int> team_rows, team_seq, team_blocks;
   std::vector<TeamBlk> team_desc;  // team_seq with each block's row fields
   TeamBlk* d_team_desc = nullptr;
@@ -902,7 +905,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     for (const Segment& s : bands[i].segs) p->segs.push_back({s.lo, s.hi, s.step, 0});
     int log2M = find2power(std::max(bands[i].band, 1));
     log2M = std::min(log2M, p->log2N);
-    if (log2M > kMaxLog2B && team_ok && r.seg_count <= kTeamSegLds) {
+    // the tuned crossover moves M = 1024 bands to the team kernel (A = 4)
+    const int direct_max = tn.direct_max == 9 ? 9 : kMaxLog2B;
+    if (log2M > direct_max && log2M > kTeamLog2B && team_ok && r.seg_count <= kTeamSegLds) {
       r.log2B = kTeamLog2B;
       r.log2A = log2M - kTeamLog2B;
       team.push_back(i);
@@ -947,7 +952,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     // item-block sequence per team: step s = pass 1 of row s, then pass 2 of
     // row s - lag; a pass 1 into buffer b waits for the pass 2 of the row
     // nbuf earlier, which sits nbuf - lag steps back
-    const int lag = std::max(1, p->team_nbuf / 2);
+    int lag = std::max(1, p->team_nbuf / 2);
+    if (tn.team_lag >= 1 && tn.team_lag < p->team_nbuf) lag = tn.team_lag;
+    p->team_lag = lag;
     p->seq_stride = 2 * per;
     p->team_seq.assign((size_t)p->seq_stride * nt, 0);
     p->team_blocks.assign(nt, 0);
@@ -996,6 +1003,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     // pass 1 of batch k runs in the launch of pass 2 of batch k - 1 (at
     // F = 2^14 a pass-1 work-group fills a CU's LDS, so the passes alternate)
     p->cols_sets = (p->cols_batches.size() > 1 && team_log2F < 14) ? 2 : 1;
+    if (tn.cols_sets == 1) p->cols_sets = 1;
   }
   // direct items, grouped by band size (one kernel instance per size)
   for (int lm = 0; lm <= kMaxLog2B; ++lm) {
@@ -2229,17 +2237,41 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
   best.team_buffers = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
   best.cols_mb = (int)(kColsScratchBytes >> 20);
   best.path = 0;  // the built-in routing unless a timed candidate wins
-  float best_ms = -1.f;
+  float best_ms = -1.f, default_ms = -1.f;
   if (effort > 0 && (teams || cols)) {
     // candidates: the team kernel's buffer counts, and the column path's
     // scratch per batch (beside the team kernel where it runs too)
+    // candidates: the team kernel's buffer count, the lag between a row's
+    // two passes and the direct/team crossover; the column path's scratch
+    // per batch and whether its passes share launches
     std::vector<Tuned> cand;
+    auto add = [&](int b, int mb, int path, int lag, int dmax, int sets) {
+      Tuned t;
+      t.team_buffers = b;
+      t.cols_mb = mb;
+      t.path = path;
+      t.team_lag = lag;
+      t.direct_max = dmax;
+      t.cols_sets = sets;
+      cand.push_back(t);
+    };
     if (teams)
-      for (int b : {2, 3, 4, 6, 8, 12})
-        if (effort == 2 || b <= 6) cand.push_back({b, best.cols_mb, 1});
+      for (int b : {2, 3, 4, 6, 8, 12}) {
+        if (effort < 2 && (b < 4 || b > 8)) continue;
+        for (int lag : {b / 2, b / 2 - 1, b / 2 + 1})
+          if (lag >= 1 && lag < b && (effort == 2 || lag == b / 2 || b == 6))
+            for (int dmax : {10, 9})
+              if (effort == 2 || dmax == 10 || (b == 6 && lag == 3)) add(b, best.cols_mb, 1, lag, dmax, 0);
+      }
     for (int mb : {32, 64, 128, 256, 512})
-      if (effort == 2 || (mb >= 64 && mb <= 256)) cand.push_back({best.team_buffers, mb, 2});
+      if (effort == 2 || (mb >= 64 && mb <= 256))
+        for (int sets : {0, 1})
+          if (effort == 2 || sets == 0 || mb == 128) add(best.team_buffers, mb, 2, 0, 0, sets);
     const int reps = effort == 1 ? 3 : 7;  // timed runs per candidate (the fastest counts)
+    // the built-in schedule first: it is kept unless a candidate beats it
+    rc = tune_time(N, threads, Tuned{}, reps, &default_ms);
+    if (rc != FCWT_HIP_OK) return rc;
+    best_ms = default_ms;
     for (const Tuned& c : cand) {
       float ms = 0.f;
       rc = tune_time(N, threads, c, reps, &ms);
@@ -2254,7 +2286,9 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   f << "fcwt_hip tuning 1\narch gfx950\nN " << N << "\nthreads " << threads << "\neffort " << effort
     << "\nteam_buffers " << best.team_buffers << "\ncols_mb " << best.cols_mb << "\npath "
-    << (best.path == 1 ? "team" : best.path == 2 ? "cols" : "default") << "\ntime_ms " << best_ms << "\n";
+    << (best.path == 1 ? "team" : best.path == 2 ? "cols" : "default") << "\nteam_lag " << best.team_lag
+    << "\ndirect_max " << best.direct_max << "\ncols_sets " << best.cols_sets << "\ntime_ms " << best_ms
+    << "\ndefault_ms " << default_ms << "\n";
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = best;
@@ -2279,11 +2313,16 @@ int fcwt_hip_tune_load(const char* path) {
     else if (key == "team_buffers") is >> t.team_buffers;
     else if (key == "cols_mb") is >> t.cols_mb;
     else if (key == "path") is >> pth;
+    else if (key == "team_lag") is >> t.team_lag;
+    else if (key == "direct_max") is >> t.direct_max;
+    else if (key == "cols_sets") is >> t.cols_sets;
   }
   t.path = pth == "team" ? 1 : pth == "cols" ? 2 : 0;
   if (arch != "gfx950") return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: tuned for another GPU: ") + path);
   if (N < 2 || (N & (N - 1)) || (t.team_buffers && (t.team_buffers < 2 || t.team_buffers > kTeamBuffers)) ||
-      t.cols_mb < 0 || t.cols_mb > 4096 || !(pth.empty() || pth == "team" || pth == "cols" || pth == "default"))
+      t.cols_mb < 0 || t.cols_mb > 4096 || !(pth.empty() || pth == "team" || pth == "cols" || pth == "default") ||
+      t.team_lag < 0 || (t.team_buffers && t.team_lag >= t.team_buffers) ||
+      !(t.direct_max == 0 || t.direct_max == 9 || t.direct_max == 10) || t.cols_sets < 0 || t.cols_sets > 2)
     return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: bad contents: ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = t;
