@@ -376,8 +376,9 @@ TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --ker
 
 # kernel traces summarized under profiles/: trace directory, file tag, title, bench.py arguments, note
 TRACES = [
-    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 20 --warmup 3 --no-cpu",
-     "Calls include the 3 warm-up steps and 20 timed steps of bench.py."),
+    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 20 --warmup 3 --no-cpu --no-tune",
+     "Calls include the 3 warm-up steps, the 20 timed steps and the 10 steps of the per-kernel event pass of "
+     "bench.py."),
     ("trace_long", "cwt_long_4194304_x_1024", "long signal, n = 2^22 x 1024 LINFREQS (sigma 4)",
      "--workload long --steps 2 --warmup 1",
      "Calls include the warm-up step, the 2 timed steps and the profiled step of bench.py."),
@@ -385,6 +386,9 @@ TRACES = [
      "--workload batched --steps 2 --warmup 1",
      "Calls include the plain and batched runs of bench.py (warm-up and timed; the batch warms up with two "
      "executes, the second of which captures its HIP graph)."),
+    ("trace_adjoint", "adjoint_64_x_16384_x_128", "adjoint, 64 signals x 16,384 x 128 LOGSCALES (sigma 2)",
+     "--workload adjoint --steps 3 --warmup 1",
+     "Calls include the plain forward, the batched forward and the adjoint runs of bench.py (warm-up and timed)."),
 ]
 
 
@@ -478,7 +482,7 @@ def summarize_sq(root):
     out = {"workload": WORKLOAD_TAG, "N": N, "n": n, "kernels": {}}
     cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {c} --output-format csv "
            "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py --steps 2 "
-           "--warmup 1 --no-cpu")
+           "--warmup 1 --no-cpu --no-tune")
     out["commands"] = [cmd.format(c=" ".join(v), d=k) for k, v in SQ_PASSES.items()]
     for kname, kind in SQ_KERNELS:
         if kname not in per:
@@ -560,8 +564,8 @@ def summarize_pmc(root):
         "fetch_scale": f_fetch, "write_scale": f_write,
         "fetch_bytes_per_launch": fetch_b, "write_bytes_per_launch": write_b,
         "traffic_bytes_per_launch": fetch_b + write_b,
-        "commands": [cmd.format(c="FETCH_SIZE", d="pmc_fetch", a="--steps 2 --warmup 1 --no-cpu"),
-                     cmd.format(c="WRITE_SIZE", d="pmc_write", a="--steps 2 --warmup 1 --no-cpu"),
+        "commands": [cmd.format(c="FETCH_SIZE", d="pmc_fetch", a="--steps 2 --warmup 1 --no-cpu --no-tune"),
+                     cmd.format(c="WRITE_SIZE", d="pmc_write", a="--steps 2 --warmup 1 --no-cpu --no-tune"),
                      cmd.format(c="FETCH_SIZE", d="copy_fetch", a="--copy-probe 1024"),
                      cmd.format(c="WRITE_SIZE", d="copy_write", a="--copy-probe 1024")],
         "summarized_by": "python bench.py --summarize-profiles gpurun_out/prof",
