@@ -554,7 +554,7 @@ def summarize_pmc(root):
     f_write = copy_bytes / (cw * 1024.0)
     fetch_b = kf * 1024.0 * f_fetch
     write_b = kw * 1024.0 * f_write
-    cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 60 rocprofv3 --pmc {c} --output-format csv "
+    cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {c} --output-format csv "
            "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py {a}")
     summary = {
         "kernel": "fcwt_hip::k_rows_team", "kind": kind, "workload": WORKLOAD_TAG,
