@@ -8,7 +8,10 @@ output 3000 x 100,000 complex64 (2.4 GB) written to HBM.
 A step is one full cwt (forward FFT + every scale) on device pointers: the
 input is already in HBM and the output buffer preallocated there.  The value
 is output GB/s = 8 * fn * n bytes / step time (HIP events around the K steps,
-no per-kernel events inside them).
+no per-kernel events inside them).  The default is K = 200 timed steps after
+W = 50 untimed ones: the GPU's clocks take some 20-30 steps (about 40 ms of
+work) to settle, and a run of 20 steps after 3 reads 1.38 ms per step where
+the settled rate is 1.25 ms.
 
 The roofline entry is for the dominant kernel (the one with the most time):
 its algorithmic bytes per launch (8 B per complex64 output value it writes,
@@ -376,14 +379,14 @@ TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --ker
 
 # kernel traces summarized under profiles/: trace directory, file tag, title, bench.py arguments, note
 TRACES = [
-    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 20 --warmup 3 --no-cpu --no-tune",
-     "Calls include the 3 warm-up steps, the 20 timed steps and the 10 steps of the per-kernel event pass of "
+    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 100 --warmup 50 --no-cpu --no-tune",
+     "Calls include the 50 warm-up steps, the 100 timed steps and the 10 steps of the per-kernel event pass of "
      "bench.py."),
     ("trace_long", "cwt_long_4194304_x_1024", "long signal, n = 2^22 x 1024 LINFREQS (sigma 4)",
-     "--workload long --steps 2 --warmup 1",
-     "Calls include the warm-up step, the 2 timed steps and the profiled step of bench.py."),
+     "--workload long --steps 3 --warmup 2",
+     "Calls include the 2 warm-up steps, the 3 timed steps and the profiled step of bench.py."),
     ("trace_batched", "cwt_batched_384_channels", "384 channels x 200 LOGSCALES (sigma 6)",
-     "--workload batched --steps 2 --warmup 1",
+     "--workload batched --steps 3 --warmup 2",
      "Calls include the plain and batched runs of bench.py (warm-up and timed; the batch warms up with two "
      "executes, the second of which captures its HIP graph)."),
     ("trace_adjoint", "adjoint_64_x_16384_x_128", "adjoint, 64 signals x 16,384 x 128 LOGSCALES (sigma 2)",
@@ -596,8 +599,8 @@ def summarize_pmc(root):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-rows", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tuned-schedule comparison")
@@ -616,10 +619,12 @@ def main():
         return
     if args.workload != "headline":
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        # these steps take 1-90 ms each: fewer of them, with enough untimed
+        # work before them for the clocks to settle
         if "--steps" not in sys.argv:
-            args.steps = 3
+            args.steps = 5
         if "--warmup" not in sys.argv:
-            args.warmup = 1
+            args.warmup = 3
         res = {"long": workload_long, "batched": workload_batched, "adjoint": workload_adjoint,
                "host": workload_host}[args.workload](args)
         print(json.dumps(res))
