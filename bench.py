@@ -325,11 +325,15 @@ KIND_KERNELS = {
 WORKLOAD_TAG = "100k_x_3000"
 
 
-def kind_bytes(kind, n, N, split):
-    """Algorithmic HBM bytes one launch of `kind` must move in this workload."""
+def kind_bytes(kind, n, N, split, direct_in_team=False):
+    """Algorithmic HBM bytes one launch of `kind` must move in this workload.
+    direct_in_team: the direct rows ran inside the team launch (its members
+    take them once their team sequence is claimed; the plan then times no
+    direct-row launch)."""
     if kind == "forward_fft":
         return 4.0 * n + 4.0 * N  # signal read + N/2 complex64 spectrum written
-    rows = {"row_writer_direct": split["direct"], "row_team": split["team"],
+    rows = {"row_writer_direct": split["direct"],
+            "row_team": split["team"] + (split["direct"] if direct_in_team else 0),
             "row_writer_two_pass": split["two_pass"], "row_columns": split["two_pass"]}.get(kind, 0)
     return 8.0 * n * rows
 
@@ -709,8 +713,11 @@ def main():
     out_bytes = 8.0 * fn * n
     value = out_bytes / (ms_step * 1e-3) / 1e9
     kernels = {}
+    # every direct row of this workload has a band of 2^9 or 2^10 bins: with a
+    # team launch they all run inside it
+    direct_in_team = "row_writer_direct" not in prof and "row_team" in prof
     for k, (ms, c) in prof.items():
-        b = kind_bytes(k, n, plan.N, split)
+        b = kind_bytes(k, n, plan.N, split, direct_in_team)
         kernels[k] = {"ms_per_launch": ms / c, "launches_per_step": c / prof_steps,
                       "bytes_per_launch": b, "gbs": b / (ms / c * 1e-3) / 1e9}
     dom = max(prof, key=lambda k: prof[k][0])
@@ -732,7 +739,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "cwt 100k x 3000: n=100000 (N=131072) float32, 5 tones + N(0,0.2) noise, fs=1000, "
                                "Morlet sigma=2, 3000 LOGSCALES 1-450 Hz, complex64 output 2.4 GB",
-                   "n": n, "fn": fn, "N": plan.N, "rows": split,
+                   "n": n, "fn": fn, "N": plan.N, "rows": split, "direct_rows_in_team_launch": direct_in_team,
                    "samples_scales_per_s": round(fn * n / (ms_step * 1e-3), 1),
                    "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
                    "tuned_schedule": tuned},

@@ -137,6 +137,16 @@ static int direct_phases(int log2M) {
     default: return DirectCfg<10>::G;
   }
 }
+// the same for a block run by the team kernel (kFillMinLog2M <= log2M <= 10)
+static int direct_phases_team(int log2M) {
+  switch (log2M) {
+    case 6: return DirectCfg<6, true>::G;
+    case 7: return DirectCfg<7, true>::G;
+    case 8: return DirectCfg<8, true>::G;
+    case 9: return DirectCfg<9, true>::G;
+    default: return DirectCfg<10, true>::G;
+  }
+}
 
 // Z[q] = X[q] * mother[(int)fminf(N-1, step*q)] for the segment holding q
 // (fcwt.cpp:224-248), 0 outside every segment.
@@ -677,8 +687,11 @@ struct fcwt_hip_plan_s {
   struct Group {
     int log2M;
     size_t begin, count;
+    bool fill;  // run by the team kernel's members (TeamArgs::fill_items), not by a launch of its own
   };
   std::vector<Group> direct_groups;
+  size_t fill_begin = 0, fill_count = 0;  // the fill groups' items, contiguous in items
+  int fill_end[kFillGroups] = {};
   struct Batch {
     size_t s_begin, s_count, w_begin, w_count;
   };
@@ -877,6 +890,23 @@ static bool cols_disabled() {
   const char* e = std::getenv("FCWT_HIP_NO_COLS");
   return e && e[0] == '1';
 }
+// Whether the direct rows (bands 2^kFillMinLog2M .. 2^10) of a plan with a
+// team launch run inside it.  Inside, they run at the team kernel's
+// occupancy (3 work-groups per CU, against 4 waves per SIMD in their own
+// launches) and, at M = 1024, in 64-byte output runs, but they fill the time
+// the team's last rows leave idle and save their own launches' ramps.  That
+// pays while the team launch is short: measured, 12-26% faster steps with
+// team_rows * N up to 2^25 (e.g. 200 scales at n = 20k..100k), even at 3e7,
+// 4% slower at 1e8 and 3e8 (1000 and 3000 scales at n = 100k).
+// FCWT_HIP_FILL=1 / FCWT_HIP_NO_FILL=1 force either way.
+constexpr double kFillMaxTeamPoints = 33554432.0;  // 2^25
+static bool fill_rows(size_t team_rows, int N) {
+  const char* off = std::getenv("FCWT_HIP_NO_FILL");
+  if (off && off[0] == '1') return false;
+  const char* on = std::getenv("FCWT_HIP_FILL");
+  if (on && on[0] == '1') return team_rows > 0;
+  return team_rows > 0 && (double)team_rows * (double)N <= kFillMaxTeamPoints;
+}
 
 static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
   const bool avx = !(p->flags & FCWT_HIP_COVERAGE_SCALAR);
@@ -1005,17 +1035,27 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     p->cols_sets = (p->cols_batches.size() > 1 && team_log2F < 14) ? 2 : 1;
     if (tn.cols_sets == 1) p->cols_sets = 1;
   }
-  // direct items, grouped by band size (one kernel instance per size)
+  // direct items, grouped by band size (one kernel instance per size); with
+  // a team launch, the bands of 2^kFillMinLog2M and up run inside it, after
+  // each member's team sequence (TeamArgs::fill_items)
+  const bool fill = p->nteams > 0 && fill_rows(team.size(), p->N);
+  p->fill_begin = p->fill_count = 0;
   for (int lm = 0; lm <= kMaxLog2B; ++lm) {
+    const bool f = fill && lm >= kFillMinLog2M;
     size_t begin = p->items.size();
+    if (f && p->fill_count == 0) p->fill_begin = begin;
     int L = 1 << (p->log2N - lm);
-    int G = std::min(direct_phases(lm), L);
+    int G = std::min(f ? direct_phases_team(lm) : direct_phases(lm), L);
     for (int i = 0; i < p->fn; ++i) {
       const RowD& r = p->rows[i];
       if (r.log2A != 0 || r.log2B != lm) continue;
       for (int b = 0; b < L / G; ++b) p->items.push_back({i, b});
     }
-    if (p->items.size() > begin) p->direct_groups.push_back({lm, begin, p->items.size() - begin});
+    if (p->items.size() > begin) p->direct_groups.push_back({lm, begin, p->items.size() - begin, f});
+    if (f) {
+      p->fill_count = p->items.size() - p->fill_begin;
+      p->fill_end[lm - kFillMinLog2M] = (int)p->fill_count;
+    }
   }
   p->direct_items = p->items.size();
   if (!wide.empty()) {
@@ -1387,9 +1427,10 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
   a.log2N = p->log2N;
   a.norm = p->normalize ? 1.0f / (float)p->N : 1.0f;
   a.tw = twiddles_of(p);
-  if (p->direct_items) {
+  if (p->direct_items > p->fill_count) {
     ProfScope ps(p, 1, st);
     for (const auto& gr : p->direct_groups) {
+      if (gr.fill) continue;
       a.items = p->d_items + gr.begin;
       launch_direct(gr.log2M, (unsigned)gr.count, a, st);
     }
@@ -1410,6 +1451,9 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     ta.scratch = p->d_team_scratch;
     ta.sync = p->d_team_sync;
     ta.error = p->d_error;
+    ta.fill_items = p->d_items + p->fill_begin;
+    ta.nfill = (int)p->fill_count;
+    for (int g = 0; g < kFillGroups; ++g) ta.fill_end[g] = p->fill_end[g];
     void* args[] = {&ta};
     HIP_TRY(hipLaunchKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0, st));
   }

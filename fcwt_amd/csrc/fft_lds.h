@@ -26,6 +26,13 @@ __device__ __forceinline__ float2 gload(const gfloat2* p) {
   unsigned long long u = *reinterpret_cast<const gu64*>(p);
   return make_float2(__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32)));
 }
+// p[0] = a, p[1] = b in one 16-byte store (p 16-byte aligned)
+__device__ __forceinline__ void gstore2(gfloat2* p, float2 a, float2 b) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  typedef __attribute__((address_space(1))) u32x4 gu32x4;
+  const u32x4 u = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(b.x), __float_as_uint(b.y)};
+  *reinterpret_cast<gu32x4*>(p) = u;
+}
 // the same with the non-temporal hint, for output that is written once and not re-read
 __device__ __forceinline__ void gstore_nt(gfloat2* p, float2 v) {
   __builtin_nontemporal_store(((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x),
@@ -44,6 +51,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(float2* p, long long 
 __device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned idx, float2 v) {
   const u32x2 u = {__float_as_uint(v.x), __float_as_uint(v.y)};
   __builtin_amdgcn_raw_buffer_store_b64(u, r, idx * 8u, 0, 2);
+}
+// row[idx] = a, row[idx + 1] = b in one 16-byte store (idx even, the row
+// 16-byte aligned; both in range or both out: the caller keeps pairs from
+// straddling the end)
+__device__ __forceinline__ void bstore2_nt(__amdgpu_buffer_rsrc_t r, unsigned idx, float2 a, float2 b) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 u = {__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(b.x), __float_as_uint(b.y)};
+  __builtin_amdgcn_raw_buffer_store_b128(u, r, idx * 8u, 0, 2);
 }
 
 constexpr int kLdsCap = 16384;  // complex64 elements per work-group (128 KiB)

@@ -26,13 +26,16 @@
 
 namespace fcwt_hip {
 
-template <int LOG2M>
+// TEAM: the configuration of a block run inside the wide-band team kernel
+// (rows_team.h), whose work-groups have 256 threads: at M = 1024 a block
+// takes 8 phases there (64-byte output runs) instead of 16.
+template <int LOG2M, bool TEAM = false>
 struct DirectCfg {
   static constexpr int M = 1 << LOG2M;
   static constexpr int R1 = M < 32 ? M : 32;
   static constexpr int P2 = M / R1;
   // phases per work-group (upper bound; fewer when the row has fewer phases)
-  static constexpr int G = P2 >= 16 ? 16 : 256 / P2;
+  static constexpr int G = P2 >= 16 ? (TEAM && P2 > 16 ? 256 / P2 : 16) : 256 / P2;
   static constexpr int NT = G * P2;                // threads
   static constexpr int XS = M + P2;                // LDS stride between phases in the exchange
   static constexpr int OS = G + 1;                 // LDS stride between m rows in the transpose
@@ -45,16 +48,16 @@ struct DirectCfg {
   static constexpr int LDS = P2 > 1 ? (BUFH > TABLES ? BUFH : TABLES) : BUF + TABLES;
 };
 
-template <int LOG2M>
-__global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
-  using C = DirectCfg<LOG2M>;
+// One block of a row: G phases of item `it`, by C::NT threads (the whole
+// work-group), in `lds` (C::LDS complex64 values).
+template <int LOG2M, bool TEAM>
+__device__ __forceinline__ void direct_block(const KArgs& a, const Item it, float2* lds) {
+  using C = DirectCfg<LOG2M, TEAM>;
   constexpr int M = C::M, R1 = C::R1, P2 = C::P2, NT = C::NT;
-  __shared__ float2 lds[C::LDS];
   float2* const buf = lds;
   float2* const zs = lds + C::BUF;     // Z[0, M)
   float2* const wm = zs + M;           // [k1][j] = W_M^{j k1}
   float2* const pre = wm + M;          // [k][g] = W_{R1 L}^{k l}
-  const Item it = a.items[blockIdx.x];
   const RowD r = a.rows[it.row];
   const int log2L = a.log2N - LOG2M;
   const int L = 1 << log2L;
@@ -209,6 +212,12 @@ __global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
       if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[m * C::OS + gg]);
     }
   }
+}
+
+template <int LOG2M>
+__global__ __launch_bounds__(DirectCfg<LOG2M>::NT) void k_rows_direct(KArgs a) {
+  __shared__ float2 lds[DirectCfg<LOG2M>::LDS];
+  direct_block<LOG2M, false>(a, a.items[blockIdx.x], lds);
 }
 
 }  // namespace fcwt_hip

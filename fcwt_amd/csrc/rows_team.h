@@ -59,6 +59,8 @@ constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
 constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths
 constexpr long kTeamSpin = 1L << 22;           // default polls (with s_sleep) before a wait is declared failed
+constexpr int kFillMinLog2M = 6;               // smallest band of a direct row the team kernel runs
+constexpr int kFillGroups = 10 - kFillMinLog2M + 1;  // bands 2^6 .. 2^10
 
 // One block of a team's item sequence with what its items read about their
 // row, so that an item starts after one burst of scalar loads.
@@ -84,11 +86,20 @@ struct TeamArgs {
   float2* scratch;         // [team][nbuf][N]
   unsigned* sync;          // zeroed before launch; see below
   unsigned* error;         // sticky report: ORed with the launch's abort code, cleared by the host when read
+  // Blocks of the rows whose band fits one work-group (2^kFillMinLog2M <=
+  // M <= 2^kMaxLog2B), run by direct_block once a member's team sequence is
+  // claimed to its end: fill_items[0, nfill), those with band 2^(kFillMinLog2M
+  // + g) before fill_end[g].
+  const Item* fill_items;
+  int nfill;
+  int fill_end[kFillGroups];
 };
 // sync layout (unsigned, 32 per 128-byte line):
+//   [0] the next fill block,
 //   [32] this launch's abort code, [33] work-groups that have finished,
 //   team t: arrive[b] at 64 * (1 + t * kTeamBuffers + b), done[b] 32 later;
 //   team t's next-item counter at 64 * (1 + nteams * kTeamBuffers + t).
+constexpr int kSyncFill = 0;
 constexpr int kSyncAbort = 32;
 constexpr int kSyncExited = 33;
 
@@ -367,18 +378,20 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
       }
   }
   __syncthreads();
-  // every LDS read first, then the stores
+  // every LDS read first, then the stores: pairs of consecutive qb (Qb >= 8
+  // is even, so a pair never straddles two f), 16 bytes per store
   constexpr int S = kTeamPoints / kTeamThreads;
   float2 o[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int e = s * kTeamThreads + threadIdx.x;
-    o[s] = buf[(e >> log2Qb) * TS + (e & ((1 << log2Qb) - 1))];
+  for (int s = 0; s < S / 2; ++s) {
+    const int e = 2 * (s * kTeamThreads + threadIdx.x);
+    o[2 * s] = buf[(e >> log2Qb) * TS + (e & ((1 << log2Qb) - 1))];
+    o[2 * s + 1] = buf[(e >> log2Qb) * TS + (e & ((1 << log2Qb) - 1)) + 1];
   }
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int e = s * kTeamThreads + threadIdx.x;
-    gstore(U + (e >> log2Qb) * kTeamB + qb0 + (e & ((1 << log2Qb) - 1)), o[s]);
+  for (int s = 0; s < S / 2; ++s) {
+    const int e = 2 * (s * kTeamThreads + threadIdx.x);
+    gstore2(U + (e >> log2Qb) * kTeamB + qb0 + (e & ((1 << log2Qb) - 1)), o[2 * s], o[2 * s + 1]);
   }
 }
 
@@ -420,23 +433,49 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
   // every thread's reads of U have landed (they fed the exchange above), so
   // the buffer may be reused; the output stores need not complete
   if (threadIdx.x == 0) add_agent(done, 1u);
+  // outputs m >= mlim have t = f + F m >= n for every f of the item: they
+  // are neither moved through LDS nor stored
+  const int mlim = __builtin_amdgcn_readfirstlane((n - f0 + (1 << LOG2F) - 1) >> LOG2F);
+  const int slim = (mlim + 15) >> 4;  // rows m of the tile in use, in steps of 16
 #pragma unroll
-  for (int k2 = 0; k2 < P2; ++k2) buf[(j + R1 * k2) * OS + g] = w[k2];
+  for (int k2 = 0; k2 < P2; ++k2)
+    if (k2 < slim) buf[(j + R1 * k2) * OS + g] = w[k2];
   __syncthreads();
   // every LDS read first, then the stores, which the row's buffer range
-  // clips at t >= n
+  // clips at t >= n; pairs of consecutive f (16-byte stores) when the row
+  // is 16-byte aligned and even in length, so that no pair straddles t = n
   constexpr int S = (kTeamB * 16) / kTeamThreads;
   float2 o[S];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int e = s * kTeamThreads + threadIdx.x;
-    o[s] = buf[(e >> 4) * OS + (e & 15)];
-  }
   const __amdgpu_buffer_rsrc_t rows = row_rsrc((float2*)out, n);
+  if (((n & 1) | ((unsigned)(size_t)out & 15u)) == 0) {
+    // pair e = s * 256 + tid: m = e >> 3 (32 s <= m < 32 s + 32), f = f0 + 2 (e & 7);
+    // rows m in [16 slim, mlim rounded up to 32) hold stale values whose
+    // stores fall at t >= n and are dropped
+    const int plim = (mlim + 31) >> 5;
 #pragma unroll
-  for (int s = 0; s < S; ++s) {
-    const int e = s * kTeamThreads + threadIdx.x;
-    bstore_nt(rows, (unsigned)(f0 + (e & 15) + ((e >> 4) << LOG2F)), o[s]);
+    for (int s = 0; s < S / 2; ++s) {
+      const int e = s * kTeamThreads + threadIdx.x;
+      if (s < plim) {
+        o[2 * s] = buf[(e >> 3) * OS + 2 * (e & 7)];
+        o[2 * s + 1] = buf[(e >> 3) * OS + 2 * (e & 7) + 1];
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < S / 2; ++s) {
+      const int e = s * kTeamThreads + threadIdx.x;
+      if (s < plim) bstore2_nt(rows, (unsigned)(f0 + 2 * (e & 7) + ((e >> 3) << LOG2F)), o[2 * s], o[2 * s + 1]);
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int e = s * kTeamThreads + threadIdx.x;
+      if (s < slim) o[s] = buf[(e >> 4) * OS + (e & 15)];
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const int e = s * kTeamThreads + threadIdx.x;
+      if (s < slim) bstore_nt(rows, (unsigned)(f0 + (e & 15) + ((e >> 4) << LOG2F)), o[s]);
+    }
   }
 }
 
@@ -456,6 +495,39 @@ __device__ __forceinline__ void team_pass1_for(int log2A, int qb0, gfloat2* U) {
     default:
       if constexpr (LOG2F > 8) team_pass1<8, LOG2F>(qb0, U);
       break;
+  }
+}
+
+// One block of a direct row (rows_direct.h) in the team kernel's LDS buffer.
+template <int LOG2M>
+__device__ __noinline__ void team_direct(KArgs a, Item it) {
+  static_assert(DirectCfg<LOG2M, true>::NT == kTeamThreads && DirectCfg<LOG2M, true>::LDS <= kTeamBuf,
+                "a direct block must fit the team kernel's work-group");
+  direct_block<LOG2M, true>(a, it, tm_buf);
+}
+
+// Direct-row blocks claimed from the launch-wide counter until none is left.
+// The team's last rows leave members idle (pass 2 of the last rows waits for
+// their pass 1, and the teams finish at different times); these blocks,
+// which depend on nothing, fill that time.
+__device__ __forceinline__ void team_fill(const TeamArgs& ta, unsigned* next, int* s_next) {
+  const int e0 = ta.fill_end[0], e1 = ta.fill_end[1], e2 = ta.fill_end[2], e3 = ta.fill_end[3];
+  for (;;) {
+    __syncthreads();  // every thread has read s_next
+    if (threadIdx.x == 0) *s_next = (int)atomicAdd(next, 1u);
+    __syncthreads();
+    const int d = __builtin_amdgcn_readfirstlane(*s_next);
+    if (d >= ta.nfill) break;
+    const int* const w = (const int*)(ta.fill_items + d);
+    const Item it = {ld_const(w), ld_const(w + 1)};
+    const int lm = kFillMinLog2M + (d >= e0) + (d >= e1) + (d >= e2) + (d >= e3);
+    switch (lm) {
+      case 6: team_direct<6>(ta.k, it); break;
+      case 7: team_direct<7>(ta.k, it); break;
+      case 8: team_direct<8>(ta.k, it); break;
+      case 9: team_direct<9>(ta.k, it); break;
+      default: team_direct<10>(ta.k, it); break;
+    }
   }
 }
 
@@ -616,6 +688,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(s_next);
   }
+  if (ta.nfill > 0) team_fill(ta, sync + kSyncFill, &s_next);
   team_exit(sync, error, ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, log2I, ta.k.out, ta.k.outs, &s_flag);
 }
 

@@ -103,10 +103,11 @@ def test_headline_100k_x_3000_rows():
 
 
 @pytest.mark.parametrize("teams", [True, False])
-@pytest.mark.parametrize("n", [20_000, 50_000, 100_000])
+@pytest.mark.parametrize("n", [20_000, 50_000, 50_001, 100_000])
 def test_wide_band_paths(n, teams, monkeypatch):
     """N = 2^15, 2^16, 2^17 with bands up to N/2, through the XCD-team kernel
-    and (with the team path switched off) the two-pass fallback."""
+    and (with the team path switched off) the two-pass fallback; an odd n
+    takes the team kernel's single-value output stores."""
     if not teams:
         monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
     x = tones_noise(n, seed=n)
@@ -136,6 +137,62 @@ def test_team_buffer_counts(nbuf, monkeypatch):
         rows = sorted(set(range(0, 120, 7)) | {119})
         ref = O.cwt_fp64(x, sc, 2.0, rows=rows)
         assert O.rel_l2_rows(got[rows], ref).max() <= TOL
+
+
+def test_direct_rows_inside_team_launch(monkeypatch):
+    """With wide rows on the XCD-team kernel, the rows whose band fits one
+    work-group (M = 2^6 .. 2^10) run inside that launch as its members' last
+    work, with no direct-row launch of their own: bit-identical to the same
+    rows through their own launches (FCWT_HIP_NO_FILL=1), and within the bar
+    against the oracle for one row of every band size."""
+    monkeypatch.setenv("FCWT_HIP_FILL", "1")
+    x = tones_noise(100_000, seed=77)
+    sc = O.scales(O.LOGSCALES, 1000, 0.2, 450, 200)
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    split = p.row_split()
+    assert split["team"] > 0 and split["direct"] > 0
+    N = 1 << O.find2power(x.size)
+    E, _, _ = O.rows_meta(N, sc, 1, True)
+    log2M = np.ceil(np.log2(np.maximum(E, 1))).astype(int)
+    assert set(range(6, 11)) <= set(log2M.tolist())  # every band size the launch takes
+    p.profile(True)
+    got = run_device(x, sc, 2.0, plan=p)
+    kinds = p.profile_read()
+    p.profile(False)
+    assert "row_writer_direct" not in kinds and "row_team" in kinds, kinds
+    monkeypatch.delenv("FCWT_HIP_FILL")
+    monkeypatch.setenv("FCWT_HIP_NO_FILL", "1")
+    q = fcwt_amd.Plan(x.size, sc, 2.0)
+    q.profile(True)
+    sep = run_device(x, sc, 2.0, plan=q)
+    assert "row_writer_direct" in q.profile_read()
+    assert np.array_equal(got, sep)
+    rows = sorted({int(np.argmax(log2M == m)) for m in set(log2M.tolist())})
+    ref = O.cwt_fp64(x, sc, 2.0, rows=rows)
+    assert O.rel_l2_rows(got[rows], ref).max() <= TOL
+    # the second execute reuses the launch's fill counter
+    assert np.array_equal(run_device(x, sc, 2.0, plan=p), got)
+
+
+def test_team_rows_unaligned_output():
+    """An output array that is only 8-byte aligned: the team kernel then
+    stores one value at a time instead of 16-byte pairs; same bits as an
+    aligned output."""
+    import torch
+
+    x = tones_noise(100_000, seed=78)
+    sc = O.scales(O.LOGSCALES, 1000, 1, 450, 120)
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    assert p.row_split()["team"] > 0
+    ref = run_device(x, sc, 2.0, plan=p)
+    dx = torch.from_numpy(x).cuda()
+    buf = torch.zeros(sc.size * x.size + 1, dtype=torch.complex64, device="cuda")
+    p.execute_device(dx.data_ptr(), buf.data_ptr() + 8, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    p.check()
+    got = buf[1:].view(sc.size, x.size).cpu().numpy()
+    assert np.array_equal(got, ref)
+    assert buf[0].item() == 0
 
 
 @pytest.mark.parametrize("fn", [1, 3, 9])
