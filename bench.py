@@ -436,6 +436,11 @@ SQ_PASSES = {
     "sq_b": ["SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_SCA", "SQ_WAIT_INST_ANY",
              "SQ_WAVE_CYCLES", "SQ_BUSY_CU_CYCLES", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"],
 }
+# wave64 VALU instructions one CU issues per clock for the row kernels' mix:
+# a probe of the 148-instruction 16-point register DFT (100 two-operand adds
+# and subtracts, 48 multiply-adds, 46 of them in the two-operand encoding)
+# measured 1.78; three-operand v_fma_f32 alone 0.98, v_add_f32 alone 1.9
+VALU_RATE = 1.78
 # kernels summarized, with the rows they write in the 100k x 3000 workload
 SQ_KERNELS = [("k_rows_team", "team"), ("k_rows_direct<9>", "direct"), ("k_rows_direct<10>", "direct")]
 
@@ -505,8 +510,9 @@ def summarize_sq(root):
             ent["salu_per_point"] = c.get("SQ_INSTS_SALU", 0) * 64 / pts
             ent["lds_per_point"] = c.get("SQ_INSTS_LDS", 0) * 64 / pts
         if ns and "SQ_INSTS_VALU" in c:
-            # one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz
-            ent["valu_floor_ns"] = c["SQ_INSTS_VALU"] / (256 * 2.4)
+            # VALU_RATE wave64 instructions per CU per clock (the measured rate of
+            # the register DFTs' mix), 256 CUs x 2.4 GHz
+            ent["valu_floor_ns"] = c["SQ_INSTS_VALU"] / (256 * 2.4 * VALU_RATE)
             ent["valu_issue_fraction"] = ent["valu_floor_ns"] / ns
         if c.get("SQ_LDS_IDX_ACTIVE"):
             ent["lds_bank_conflict_fraction"] = c.get("SQ_LDS_BANK_CONFLICT", 0) / c["SQ_LDS_IDX_ACTIVE"]
@@ -523,7 +529,9 @@ def summarize_sq(root):
         f.write("\nSummarized by `python bench.py --summarize-profiles gpurun_out/prof` (average kernel durations "
                 "from the trace/ run of `profiles/cwt_100k_x_3000_kernel_trace.md`). Counts are per dispatch, "
                 "summed over the device; per point = x 64 lanes / (rows x N), N = 131072. The VALU floor is "
-                "SQ_INSTS_VALU / (256 CUs x 2.4 GHz), one wave64 VALU instruction per CU per clock.\n\n")
+                f"SQ_INSTS_VALU / (256 CUs x 2.4 GHz x {VALU_RATE}): {VALU_RATE} wave64 VALU instructions per CU "
+                "per clock, the rate measured for the 16-point register DFT's mix of two-operand (VOP2) and "
+                "three-operand (VOP3) encodings; VALU issue = floor / duration.\n\n")
         keys = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_SMEM", "SQ_INSTS_VMEM_RD",
                 "SQ_INSTS_VMEM_WR", "SQ_INSTS_BRANCH", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU",
                 "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"]
