@@ -435,6 +435,8 @@ SQ_PASSES = {
              "SQ_INSTS_SMEM", "SQ_WAVES", "SQ_INSTS_BRANCH"],
     "sq_b": ["SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS", "SQ_ACTIVE_INST_SCA", "SQ_WAIT_INST_ANY",
              "SQ_WAVE_CYCLES", "SQ_BUSY_CU_CYCLES", "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"],
+    "sq_c": ["SQ_WAIT_ANY", "SQ_ACTIVE_INST_ANY", "SQ_WAIT_INST_LDS", "SQ_IFETCH", "SQ_INSTS_VALU_FMA_F32",
+             "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_LDS_DATA_FIFO_FULL"],
 }
 # wave64 VALU instructions one CU issues per clock for the row kernels' mix:
 # a probe of the 148-instruction 16-point register DFT (100 two-operand adds
@@ -516,6 +518,14 @@ def summarize_sq(root):
             ent["valu_issue_fraction"] = ent["valu_floor_ns"] / ns
         if c.get("SQ_LDS_IDX_ACTIVE"):
             ent["lds_bank_conflict_fraction"] = c.get("SQ_LDS_BANK_CONFLICT", 0) / c["SQ_LDS_IDX_ACTIVE"]
+        if c.get("SQ_WAVE_CYCLES"):
+            # shares of the waves' time: issuing, waiting on memory/LDS results
+            # (s_waitcnt), waiting on anything (also barriers and s_sleep)
+            wc = c["SQ_WAVE_CYCLES"]
+            for k, name in (("SQ_ACTIVE_INST_ANY", "wave_issuing"), ("SQ_WAIT_INST_ANY", "wave_waiting_results"),
+                            ("SQ_WAIT_ANY", "wave_waiting_any")):
+                if k in c:
+                    ent[name] = c[k] / wc
         out["kernels"][kname] = ent
     base = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"row_kernels_{WORKLOAD_TAG}_instruction_mix")
@@ -535,9 +545,11 @@ def summarize_sq(root):
         keys = ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_SMEM", "SQ_INSTS_VMEM_RD",
                 "SQ_INSTS_VMEM_WR", "SQ_INSTS_BRANCH", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_VALU",
                 "SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"]
+        keys += ["SQ_WAVE_CYCLES", "SQ_WAIT_ANY", "SQ_ACTIVE_INST_ANY", "SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32",
+                 "SQ_INSTS_VALU_MUL_F32"]
         f.write("| kernel | rows | avg us | " + " | ".join(keys) + " | VALU/pt | SALU/pt | LDS/pt | VALU floor us "
-                "| VALU issue | LDS conflict |\n")
-        f.write("|" + "---|" * (len(keys) + 9) + "\n")
+                "| VALU issue | LDS conflict | waves issuing | waves waiting on results | waves waiting on anything |\n")
+        f.write("|" + "---|" * (len(keys) + 12) + "\n")
         for kname, e in out["kernels"].items():
             c = e["per_dispatch"]
             cells = [f"{c[k] / 1e6:.2f} M" if k in c else "-" for k in keys]
@@ -547,7 +559,9 @@ def summarize_sq(root):
             f.write(f"| `{kname}` | {e['rows'] or '-'} | {g('avg_ns', '{:.1f}', 1e-3)} | " + " | ".join(cells)
                     + f" | {g('valu_per_point', '{:.1f}')} | {g('salu_per_point', '{:.1f}')}"
                     + f" | {g('lds_per_point', '{:.1f}')} | {g('valu_floor_ns', '{:.1f}', 1e-3)}"
-                    + f" | {g('valu_issue_fraction', '{:.2f}')} | {g('lds_bank_conflict_fraction', '{:.3f}')} |\n")
+                    + f" | {g('valu_issue_fraction', '{:.2f}')} | {g('lds_bank_conflict_fraction', '{:.3f}')}"
+                    + f" | {g('wave_issuing', '{:.2f}')} | {g('wave_waiting_results', '{:.2f}')}"
+                    + f" | {g('wave_waiting_any', '{:.2f}')} |\n")
     print(json.dumps(out)[:2000])
 
 
