@@ -732,6 +732,17 @@ def main():
                               if k in chosen}}
         tplan.close()
 
+    # what a plain write stream reaches on this GPU: torch's fill kernel over
+    # the same 2.4 GB output (reported beside the 8 TB/s peak)
+    for _ in range(3):
+        out.fill_(0)
+    e0.record(stream)
+    for _ in range(10):
+        out.fill_(0)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    fill_gbs = 10 * 8.0 * fn * n / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
     out_bytes = 8.0 * fn * n
     value = out_bytes / (ms_step * 1e-3) / 1e9
     kernels = {}
@@ -772,6 +783,7 @@ def main():
                                f"{d['bytes_per_launch'] / 1e9:.4f} GB algorithmic per launch, "
                                f"{d['ms_per_launch']:.4f} ms per launch",
                      "traffic_source": f"profiles/{dom}_{WORKLOAD_TAG}_traffic.json" if tr else None,
+                     "fill_write_gbs": round(fill_gbs, 1),
                      "kernels": kernels},
     }
     if not args.no_cpu:

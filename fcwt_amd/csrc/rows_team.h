@@ -500,10 +500,28 @@ __device__ __forceinline__ void team_pass1_for(int log2A, int qb0, gfloat2* U) {
 
 // One block of a direct row (rows_direct.h) in the team kernel's LDS buffer.
 template <int LOG2M>
-__device__ __noinline__ void team_direct(KArgs a, Item it) {
+__device__ __noinline__ void team_direct(const float2* spec, const float* mother, const SegD* segs, const RowD* rows,
+                                         float2* out, float2* const* outs, const float2* t1, const float2* t2,
+                                         int s1bits, int log2N, float norm, int row, int block) {
   static_assert(DirectCfg<LOG2M, true>::NT == kTeamThreads && DirectCfg<LOG2M, true>::LDS <= kTeamBuf,
                 "a direct block must fit the team kernel's work-group");
-  direct_block<LOG2M, true>(a, it, tm_buf);
+  KArgs a;
+  a.spec = spec;
+  a.mother = mother;
+  a.segs = segs;
+  a.rows = rows;
+  a.items = nullptr;
+  a.scratch = nullptr;
+  a.out = out;
+  a.outs = outs;
+  a.N = 1 << log2N;
+  a.log2N = log2N;
+  a.norm = norm;
+  a.tw.t1 = t1;
+  a.tw.t2 = t2;
+  a.tw.s1bits = s1bits;
+  a.tw.log2N = log2N;
+  direct_block<LOG2M, true>(a, Item{row, block}, tm_buf);
 }
 
 // Direct-row blocks claimed from the launch-wide counter until none is left.
@@ -521,13 +539,17 @@ __device__ __forceinline__ void team_fill(const TeamArgs& ta, unsigned* next, in
     const int* const w = (const int*)(ta.fill_items + d);
     const Item it = {ld_const(w), ld_const(w + 1)};
     const int lm = kFillMinLog2M + (d >= e0) + (d >= e1) + (d >= e2) + (d >= e3);
+    const KArgs& k = ta.k;
+#define FCWT_TEAM_DIRECT(L) \
+  team_direct<L>(k.spec, k.mother, k.segs, k.rows, k.out, k.outs, k.tw.t1, k.tw.t2, k.tw.s1bits, k.log2N, k.norm, it.row, it.block)
     switch (lm) {
-      case 6: team_direct<6>(ta.k, it); break;
-      case 7: team_direct<7>(ta.k, it); break;
-      case 8: team_direct<8>(ta.k, it); break;
-      case 9: team_direct<9>(ta.k, it); break;
-      default: team_direct<10>(ta.k, it); break;
+      case 6: FCWT_TEAM_DIRECT(6); break;
+      case 7: FCWT_TEAM_DIRECT(7); break;
+      case 8: FCWT_TEAM_DIRECT(8); break;
+      case 9: FCWT_TEAM_DIRECT(9); break;
+      default: FCWT_TEAM_DIRECT(10); break;
     }
+#undef FCWT_TEAM_DIRECT
   }
 }
 
