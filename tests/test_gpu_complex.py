@@ -154,3 +154,18 @@ int main() {
                     "-o", str(exe)], check=True, capture_output=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True, timeout=120).stdout.split()
     assert float(out[0]) <= 2 * TOL and float(out[1]) > 0 and out[2] == "caught"
+
+
+def test_complex_input_team_launch_with_direct_rows():
+    """Complex input at n = 30,000 (N = 2^15) with 200 scales: a plan small
+    enough that its direct rows run inside the team launch; every row
+    against the oracle."""
+    x = analytic(30_000, seed=91)
+    sc = O.scales(O.LOGSCALES, 1000, 2, 480, 200)  # every direct row's band is 2^8 .. 2^10
+    p = fcwt_amd.Plan(x.size, sc, 2.0)
+    split = p.row_split()
+    assert split["team"] > 0 and split["direct"] > 0, split
+    p.profile(True)
+    run_device_complex(x, sc, 2.0, plan=p)
+    assert "row_writer_direct" not in p.profile_read()
+    _check(x, sc)
