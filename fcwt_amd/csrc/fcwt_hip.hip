@@ -894,18 +894,20 @@ static bool cols_disabled() {
 // team launch run inside it.  Inside, they run at the team kernel's
 // occupancy (3 work-groups per CU, against 4 waves per SIMD in their own
 // launches) and, at M = 1024, in 64-byte output runs, but they fill the time
-// the team's last rows leave idle and save their own launches' ramps.  That
-// pays while the team launch is short: measured, 12-26% faster steps with
-// team_rows * N up to 2^25 (e.g. 200 scales at n = 20k..100k), even at 3e7,
-// 4% slower at 1e8 and 3e8 (1000 and 3000 scales at n = 100k).
+// the team's last rows leave idle and save their own launches' ramps.
+// Measured: one-signal plans 12-26% faster with team_rows * N up to 2^25
+// (e.g. 200 scales at n = 20k..100k), even at 3e7, 4% slower at 1e8 and 3e8
+// (1000 and 3000 scales at n = 100k); the plans of the 384-channel batch
+// (several signals each, 19.7k team rows at N = 2^17) 1.3% faster.  So
+// plans of several signals always fill, one-signal plans up to 2^25.
 // FCWT_HIP_FILL=1 / FCWT_HIP_NO_FILL=1 force either way.
 constexpr double kFillMaxTeamPoints = 33554432.0;  // 2^25
-static bool fill_rows(size_t team_rows, int N) {
+static bool fill_rows(size_t team_rows, int N, int signals) {
   const char* off = std::getenv("FCWT_HIP_NO_FILL");
   if (off && off[0] == '1') return false;
   const char* on = std::getenv("FCWT_HIP_FILL");
   if (on && on[0] == '1') return team_rows > 0;
-  return team_rows > 0 && (double)team_rows * (double)N <= kFillMaxTeamPoints;
+  return team_rows > 0 && (signals > 1 || (double)team_rows * (double)N <= kFillMaxTeamPoints);
 }
 
 static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
@@ -1038,7 +1040,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   // direct items, grouped by band size (one kernel instance per size); with
   // a team launch, the bands of 2^kFillMinLog2M and up run inside it, after
   // each member's team sequence (TeamArgs::fill_items)
-  const bool fill = p->nteams > 0 && fill_rows(team.size(), p->N);
+  const bool fill = p->nteams > 0 && fill_rows(team.size(), p->N, p->nsig);
   p->fill_begin = p->fill_count = 0;
   for (int lm = 0; lm <= kMaxLog2B; ++lm) {
     const bool f = fill && lm >= kFillMinLog2M;
