@@ -478,6 +478,23 @@ VALU_RATE = 1.78
 SQ_KERNELS = [("k_rows_team", "team"), ("k_rows_direct<9>", "direct"), ("k_rows_direct<10>", "direct")]
 
 
+def pruned_ifft_flops(n, scales, fb):
+    """5 N sum_i log2 M_i: the floating-point operations of the band-limited
+    inverse transforms (radix-2 count, M_i = 2^ceil(log2 band_i)), from the
+    library's host planning (fcwt_hip_describe_rows; no device)."""
+    import ctypes
+
+    from fcwt_amd import _native as nat
+
+    sc = np.ascontiguousarray(scales, np.float32)
+    band = np.zeros(sc.size, np.int32)
+    nat.check(nat.lib().fcwt_hip_describe_rows(int(n), nat.fptr(sc), sc.size, float(fb), 1, 0, None,
+                                                band.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), None),
+              "describe_rows")
+    N = 1 << int(np.ceil(np.log2(max(int(n), 1))))
+    return 5.0 * N * float(np.ceil(np.log2(np.maximum(band, 1))).sum())
+
+
 def headline_rows_per_kernel():
     """Rows of the 100k x 3000 plan per row kernel, from the library's host
     planning (fcwt_hip_describe_rows; no device): M = 2^ceil(log2 band) rows
@@ -794,6 +811,7 @@ def main():
                                "Morlet sigma=2, 3000 LOGSCALES 1-450 Hz, complex64 output 2.4 GB",
                    "n": n, "fn": fn, "N": plan.N, "rows": split, "direct_rows_in_team_launch": direct_in_team,
                    "samples_scales_per_s": round(fn * n / (ms_step * 1e-3), 1),
+                   "pruned_ifft_tflop_s": round(pruned_ifft_flops(n, scs.scales, fb) / (ms_step * 1e-3) / 1e12, 2),
                    "pct_hbm_peak": round(100 * value / HBM_PEAK_GBS, 2),
                    "tuned_schedule": tuned},
         "roofline": {"bound": "hbm", "achieved": round(d["gbs"], 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
