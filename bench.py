@@ -177,7 +177,8 @@ def workload_long(args):
     if not args.no_tune:
         # the analogue of create_FFT_optimization_plan for N = 2^22
         # (fcwt_hip_tune, FFTW_MEASURE effort: the column path's scratch per
-        # batch and launch fusion), then a new plan timed the same way
+        # batch, launch fusion and the split 512 x 2^13 or 256 x 2^14), then a
+        # new plan timed the same way
         chosen, tune_s = _tune(plan.N, 1)
         tplan = fcwt_amd.Plan(n, sc.scales, fb)
         tms = _timed(torch, st, lambda: tplan.execute_device(dx.data_ptr(), out.data_ptr(), st.cuda_stream),
@@ -185,7 +186,7 @@ def workload_long(args):
         tplan.check()
         tplan.close()
         tuned = {"ms_per_step": round(tms, 3), "vs_default": round(ms / tms, 4), "tune_s": round(tune_s, 2),
-                 "schedule": {k: chosen[k].strip() for k in ("cols_mb", "cols_sets", "path") if k in chosen}}
+                 "schedule": {k: chosen[k].strip() for k in ("cols_mb", "cols_sets", "cols_log2B", "path") if k in chosen}}
     gbs = 8.0 * fn * n / (ms * 1e-3) / 1e9
     return {"metric": "long-signal CWT output GB/s (n=2^22 x 1024)", "value": round(gbs, 2), "unit": "GB/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),

@@ -629,6 +629,7 @@ struct Tuned {
   int team_lag = 0;      // steps between a row's pass 1 and its pass 2 in a team's sequence (< team_buffers)
   int direct_max = 0;    // widest band (log2 M) on the single-work-group kernel: 9 or 10
   int cols_sets = 0;     // column path with several batches: 1 separate passes, 2 fused launches
+  int cols_log2B = 0;    // column path at N = 2^22: N split as 2^8 x 2^14 (8) or 2^9 x 2^13 (9)
 };
 static std::mutex g_tune_mu;
 static std::map<int, Tuned> g_tuned;
@@ -701,6 +702,7 @@ struct fcwt_hip_plan_s {
   // batches of rows cols_rows[begin, begin + count), row k of a batch in
   // scratch slot k
   int cols_log2F = 0;
+  int cols_log2B = kTeamLog2B;        // N = 2^cols_log2B x 2^cols_log2F on the column path
   int cols_slots = 0;
   std::vector<int> cols_rows;
   struct ColsBatch {
@@ -890,6 +892,18 @@ static bool cols_disabled() {
   const char* e = std::getenv("FCWT_HIP_NO_COLS");
   return e && e[0] == '1';
 }
+// log2 B of the column path's split N = B F.  B = 256, except at N = 2^22,
+// which splits as 512 x 2^13: a pass-1 work-group of 2^13 points leaves room
+// for a second one and for pass-2 work-groups on its CU, where one of 2^14
+// points fills the CU's LDS alone, and a half band of 2^13 points skips the
+// radix-2 pass.  A tuning file's cols_log2B, or FCWT_HIP_COLS_B256=1 in the
+// environment, keeps 256 x 2^14.
+static int cols_split(int log2N, const Tuned& tn) {
+  if (log2N != 22) return kTeamLog2B;
+  const char* e = std::getenv("FCWT_HIP_COLS_B256");
+  if (e && e[0] == '1') return kTeamLog2B;
+  return tn.cols_log2B == kTeamLog2B ? kTeamLog2B : 9;
+}
 // Whether the direct rows (bands 2^kFillMinLog2M .. 2^10) of a plan with a
 // team launch run inside it.  Inside, they run at the team kernel's
 // occupancy (3 work-groups per CU, against 4 waves per SIMD in their own
@@ -927,6 +941,7 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   const bool team_ok =
       p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F && tn.path != 2;
   const bool cols_ok = team_log2F >= kColsMinLog2F && team_log2F <= kColsMaxLog2F && !cols_disabled();
+  const int cols_lb = cols_split(p->log2N, tn), cols_lf = p->log2N - cols_lb;
   for (int i = 0, sgi = 0; i < p->fn; ++i) {
     while (i >= p->sig_row0[sgi] + p->sig_fn[sgi]) ++sgi;
     RowD& r = p->rows[i];
@@ -944,8 +959,8 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       r.log2A = log2M - kTeamLog2B;
       team.push_back(i);
     } else if (log2M > kMaxLog2B && cols_ok && r.seg_count <= kTeamSegLds) {
-      r.log2B = kTeamLog2B;
-      r.log2A = log2M - kTeamLog2B;
+      r.log2B = cols_lb;
+      r.log2A = log2M - cols_lb;
       cols.push_back(i);
     } else {
       r.log2B = std::min(log2M, kMaxLog2B);
@@ -1016,7 +1031,8 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   if (!cols.empty()) {
     // batches of rows, one N-point scratch slot per row; the slots of one
     // batch are meant to stay in the memory-side cache between the passes
-    p->cols_log2F = team_log2F;
+    p->cols_log2F = cols_lf;
+    p->cols_log2B = cols_lb;
     const size_t per_row = sizeof(float2) * (size_t)p->N;
     size_t budget = tn.cols_mb ? (size_t)tn.cols_mb << 20 : kColsScratchBytes;
     if (const char* e = std::getenv("FCWT_HIP_COLS_SCRATCH_MB")) {
@@ -1031,11 +1047,17 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
       p->cols_batches.push_back({(int)b0, cnt});
       for (int k = 0; k < cnt; ++k) p->rows[cols[b0 + k]].slot = k;
     }
-    // with several batches and F <= 2^13, batch k uses scratch set k & 1 and
-    // pass 1 of batch k runs in the launch of pass 2 of batch k - 1 (at
-    // F = 2^14 a pass-1 work-group fills a CU's LDS, so the passes alternate)
-    p->cols_sets = (p->cols_batches.size() > 1 && team_log2F < 14) ? 2 : 1;
+    // with several batches and F <= 2^13 at B = 256, batch k uses scratch
+    // set k & 1 and pass 1 of batch k runs in the launch of pass 2 of batch
+    // k - 1 (at F = 2^14 a pass-1 work-group fills a CU's LDS, so the passes
+    // alternate; at B = 512 they alternate too: fused launches measured
+    // 2% faster to 14% slower than separate ones, depending on the GPU)
+    p->cols_sets = (p->cols_batches.size() > 1 && cols_lf < 14 && cols_lb == kTeamLog2B) ? 2 : 1;
     if (tn.cols_sets == 1) p->cols_sets = 1;
+    if (const char* e = std::getenv("FCWT_HIP_COLS_SETS")) {  // 1: separate passes, 2: fused (several batches)
+      if (e[0] == '1') p->cols_sets = 1;
+      if (e[0] == '2' && p->cols_batches.size() > 1) p->cols_sets = 2;
+    }
   }
   // direct items, grouped by band size (one kernel instance per size); with
   // a team launch, the bands of 2^kFillMinLog2M and up run inside it, after
@@ -1369,20 +1391,30 @@ static void launch_forward(fcwt_hip_plan p, const float* const* xs, bool complex
   }
 }
 
-// One column-path launch for F = 2^LF: pass 1 of n1 rows (b1) beside pass 2
-// of n2 rows (b2); pass 2 alone (n1 = 0) runs as 256-thread work-groups.
-template <int LF>
+// One column-path launch for N = 2^LB x 2^LF: pass 1 of n1 rows (b1) beside
+// pass 2 of n2 rows (b2); pass 2 alone (n1 = 0) runs as work-groups of one
+// item (256 threads at B = 256, 512 at B = 512).
+template <int LF, int LB = kTeamLog2B>
 static void launch_cols_t(const BigArgs& b1, int n1, const BigArgs& b2, int n2, hipStream_t st) {
   using C = ColsCfg<LF>;
+  using G = ColsGrid<LF, LB>;
   if (n1 == 0) {
-    k_cols_out<LF><<<dim3((1 << LF) / 16, n2), 256, 0, st>>>(b2);
+    if constexpr (LB == 9)
+      k_cols_out512<LF><<<dim3((1 << LF) / 16, n2), 512, 0, st>>>(b2);
+    else
+      k_cols_out<LF><<<dim3((1 << LF) / 16, n2), 256, 0, st>>>(b2);
     return;
   }
-  const unsigned wgs = (unsigned)((256 >> C::LOG2QC) * n1 + ((1 << LF) / 16) / (C::T / 256) * n2);
-  k_cols_step<LF><<<wgs, C::T, 0, st>>>(b1, n1, b2, n2);
+  const unsigned wgs = (unsigned)(G::X1 * n1 + G::X2 * n2);
+  k_cols_step<LF, LB><<<wgs, C::T, 0, st>>>(b1, n1, b2, n2);
 }
 
-static void launch_cols(int log2F, const BigArgs& b1, int n1, const BigArgs& b2, int n2, hipStream_t st) {
+static void launch_cols(int log2F, int log2B, const BigArgs& b1, int n1, const BigArgs& b2, int n2,
+                        hipStream_t st) {
+  if (log2B == 9) {  // only N = 2^22 splits this way (cols_split)
+    launch_cols_t<13, 9>(b1, n1, b2, n2, st);
+    return;
+  }
   switch (log2F) {
     case 5: launch_cols_t<5>(b1, n1, b2, n2, st); break;
     case 6: launch_cols_t<6>(b1, n1, b2, n2, st); break;
@@ -1460,7 +1492,7 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     HIP_TRY(hipLaunchKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0, st));
   }
   if (!p->cols_batches.empty()) {
-    const int lf = p->cols_log2F;
+    const int lf = p->cols_log2F, lb = p->cols_log2B;
     BigArgs b;
     b.k = a;
     b.xt = p->d_xt;
@@ -1470,8 +1502,8 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
     b.mnorm = p->m_norm;
     {
       ProfScope ps(p, 7, st);
-      k_spec_transpose<<<dim3(16 * std::max(1, (1 << lf) / 128), (unsigned)p->nsig), 256, 0, st>>>(p->d_spec, p->d_xt,
-                                                                                                lf);
+      k_spec_transpose<<<dim3((1 << (lb - 4)) * std::max(1, (1 << lf) / 128), (unsigned)p->nsig), 256, 0, st>>>(
+          p->d_spec, p->d_xt, lf, lb);
     }
     const size_t set_elems = (size_t)p->cols_slots * p->N;
     const size_t nb = p->cols_batches.size();
@@ -1486,13 +1518,14 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
       // launch k: pass 1 of batch k (scratch set k & 1) beside pass 2 of batch k - 1
       for (size_t k = 0; k <= nb; ++k) {
         const BigArgs b1 = k < nb ? batch_args(k) : b, b2 = k > 0 ? batch_args(k - 1) : b;
-        launch_cols(lf, b1, k < nb ? p->cols_batches[k].count : 0, b2, k > 0 ? p->cols_batches[k - 1].count : 0, st);
+        launch_cols(lf, lb, b1, k < nb ? p->cols_batches[k].count : 0, b2, k > 0 ? p->cols_batches[k - 1].count : 0,
+                    st);
       }
     } else {
       for (size_t k = 0; k < nb; ++k) {
         const BigArgs bk = batch_args(k);
-        launch_cols(lf, bk, p->cols_batches[k].count, b, 0, st);
-        launch_cols(lf, b, 0, bk, p->cols_batches[k].count, st);
+        launch_cols(lf, lb, bk, p->cols_batches[k].count, b, 0, st);
+        launch_cols(lf, lb, b, 0, bk, p->cols_batches[k].count, st);
       }
     }
   }
@@ -2291,7 +2324,7 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
     // two passes and the direct/team crossover; the column path's scratch
     // per batch and whether its passes share launches
     std::vector<Tuned> cand;
-    auto add = [&](int b, int mb, int path, int lag, int dmax, int sets) {
+    auto add = [&](int b, int mb, int path, int lag, int dmax, int sets, int lb = 0) {
       Tuned t;
       t.team_buffers = b;
       t.cols_mb = mb;
@@ -2299,6 +2332,7 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
       t.team_lag = lag;
       t.direct_max = dmax;
       t.cols_sets = sets;
+      t.cols_log2B = lb;
       cand.push_back(t);
     };
     if (teams)
@@ -2312,7 +2346,9 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
     for (int mb : {32, 64, 128, 256, 512})
       if (effort == 2 || (mb >= 64 && mb <= 256))
         for (int sets : {0, 1})
-          if (effort == 2 || sets == 0 || mb == 128) add(best.team_buffers, mb, 2, 0, 0, sets);
+          if (effort == 2 || sets == 0 || mb == 128)
+            for (int lb : {0, kTeamLog2B})  // at N = 2^22 also the 256 x 2^14 split (cols_split)
+              if (lb == 0 || log2N == 22) add(best.team_buffers, mb, 2, 0, 0, sets, lb);
     const int reps = effort == 1 ? 3 : 7;  // timed runs per candidate (the fastest counts)
     // the built-in schedule first: it is kept unless a candidate beats it
     rc = tune_time(N, threads, Tuned{}, reps, &default_ms);
@@ -2333,7 +2369,8 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
   f << "fcwt_hip tuning 1\narch gfx950\nN " << N << "\nthreads " << threads << "\neffort " << effort
     << "\nteam_buffers " << best.team_buffers << "\ncols_mb " << best.cols_mb << "\npath "
     << (best.path == 1 ? "team" : best.path == 2 ? "cols" : "default") << "\nteam_lag " << best.team_lag
-    << "\ndirect_max " << best.direct_max << "\ncols_sets " << best.cols_sets << "\ntime_ms " << best_ms
+    << "\ndirect_max " << best.direct_max << "\ncols_sets " << best.cols_sets << "\ncols_log2B " << best.cols_log2B
+    << "\ntime_ms " << best_ms
     << "\ndefault_ms " << default_ms << "\n";
   if (!f) return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune: cannot write ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2362,13 +2399,15 @@ int fcwt_hip_tune_load(const char* path) {
     else if (key == "team_lag") is >> t.team_lag;
     else if (key == "direct_max") is >> t.direct_max;
     else if (key == "cols_sets") is >> t.cols_sets;
+    else if (key == "cols_log2B") is >> t.cols_log2B;
   }
   t.path = pth == "team" ? 1 : pth == "cols" ? 2 : 0;
   if (arch != "gfx950") return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: tuned for another GPU: ") + path);
   if (N < 2 || (N & (N - 1)) || (t.team_buffers && (t.team_buffers < 2 || t.team_buffers > kTeamBuffers)) ||
       t.cols_mb < 0 || t.cols_mb > 4096 || !(pth.empty() || pth == "team" || pth == "cols" || pth == "default") ||
       t.team_lag < 0 || (t.team_buffers && t.team_lag >= t.team_buffers) ||
-      !(t.direct_max == 0 || t.direct_max == 9 || t.direct_max == 10) || t.cols_sets < 0 || t.cols_sets > 2)
+      !(t.direct_max == 0 || t.direct_max == 9 || t.direct_max == 10) || t.cols_sets < 0 || t.cols_sets > 2 ||
+      !(t.cols_log2B == 0 || t.cols_log2B == kTeamLog2B || t.cols_log2B == 9))
     return fail(FCWT_HIP_EIO, std::string("fcwt_hip_tune_load: bad contents: ") + path);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = t;

@@ -1,0 +1,73 @@
+"""CPU emulation of the column path's pass-1 index arithmetic
+(fcwt_amd/csrc/rows_big.h: cols_skip, cols_fft, cols_pass): for every F of
+the path and bands A <= F/2, the passes that only replicate are skipped, the
+first pass run reads position p of a column as z[p >> log2S], the passes in
+between are radix-2^(log2F mod 4) then radix-16 Stockham steps with the
+kernel's twiddle exponents, and the last pass writes position
+dst + k Ns.  The result must be the inverse DFT of the band,
+sum_{q < A} z[q] e^{+2 pi i q f / F}, to float64 rounding.  No GPU."""
+import numpy as np
+import pytest
+
+
+def cols_skip(log2F, A):
+    first = log2F % 4
+    npass = (first > 0) + (log2F - first) // 4
+    skip, l2, step = 0, 0, first if first > 0 else 4
+    while skip < npass and (A << (l2 + step)) <= (1 << log2F):
+        l2 += step
+        skip += 1
+        step = 4
+    return skip, l2, npass
+
+
+def passes(log2F):
+    first = log2F % 4
+    npass = (first > 0) + (log2F - first) // 4
+    out = []
+    for j in range(npass):
+        log2R = first if (j == 0 and first > 0) else 4
+        log2NS = 0 if j == 0 else (first + 4 * (j - 1) if first > 0 else 4 * j)
+        out.append((log2R, log2NS))
+    return out
+
+
+def emulate_column(log2F, z):
+    F = 1 << log2F
+    A = z.size
+    skip, log2S, npass = cols_skip(log2F, A)
+    p = np.arange(F)
+    gq = p >> log2S
+    x = np.where(gq < A, np.concatenate([z, np.zeros(F, complex)])[np.minimum(gq, F - 1)], 0)
+    if skip >= npass:  # every pass replicates
+        return x
+    for log2R, log2NS in passes(log2F)[skip:]:
+        R, Ns = 1 << log2R, 1 << log2NS
+        log2GPC = log2F - log2R
+        gl = np.arange(1 << log2GPC)
+        k = np.arange(R)
+        v = x[gl[:, None] + (k[None, :] << log2GPC)]
+        if log2NS > 0:
+            jj = gl & (Ns - 1)
+            sh = log2F - log2NS - log2R
+            v = v * np.exp(2j * np.pi * (((jj[:, None] * k[None, :]) << sh) % F) / F)
+        v = np.fft.ifft(v, axis=1) * R  # sum_j v[j] e^{+2 pi i jk / R}
+        dst = ((gl >> log2NS) << (log2NS + log2R)) + (gl & (Ns - 1))
+        y = np.empty(F, complex)
+        y[dst[:, None] + (k[None, :] << log2NS)] = v
+        x = y
+    return x
+
+
+@pytest.mark.parametrize("log2F", range(5, 15))
+def test_pass1_transform_every_F(log2F):
+    F = 1 << log2F
+    rng = np.random.default_rng(log2F)
+    for A in sorted({1, 2, 3, 4, 8, 16, 17, 32, 100, F // 32, F // 16 + 1, F // 4, F // 2} - {0}):
+        if A > F // 2:
+            continue
+        z = rng.standard_normal(A) + 1j * rng.standard_normal(A)
+        got = emulate_column(log2F, z)
+        ref = np.fft.ifft(np.concatenate([z, np.zeros(F - A)])) * F
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, (log2F, A, err)

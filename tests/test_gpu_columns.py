@@ -1,7 +1,8 @@
 """GPU parity of the column path for wide bands (k_spec_transpose,
-k_cols_step, k_cols_out): padded sizes N = 2^13 .. 2^22 with bands up to N/2, against the
-float64 oracle, and against the generic two-pass kernels (FCWT_HIP_NO_COLS=1)
-on the same plan shape.
+k_cols_step, k_cols_out, k_cols_out512): padded sizes N = 2^13 .. 2^22 with
+bands up to N/2, against the float64 oracle, against the generic two-pass
+kernels (FCWT_HIP_NO_COLS=1) on the same plan shape, and at N = 2^22 between
+its two splits (512 x 2^13 and 256 x 2^14).
 
 Bar: relative L2 <= 1e-5 per row (TOL).  At N = 2^22 the oracle checks a
 sample of rows that spans every band size (a float64 4M-point transform per
@@ -123,6 +124,54 @@ def test_column_path_linearity_full_rows():
     yb = run_device(b, sc, 4.0, plan=p)
     yab = run_device(a + b, sc, 4.0, plan=p)
     assert O.rel_l2_rows(yab, ya + yb).max() < 3e-5
+
+
+@pytest.mark.parametrize("n,threads", [((1 << 21) + 12_345, 1), (1 << 22, 6)])
+def test_column_path_splits_at_2_22(n, threads, monkeypatch):
+    """N = 2^22 splits as 512 x 2^13 (the default) or, with
+    FCWT_HIP_COLS_B256=1, as 256 x 2^14: sampled rows of both against the
+    oracle, every row of the two against each other.  threads = 6 skips bins
+    near Nyquist, so the wide rows carry several band segments."""
+    x = tones_noise(n, seed=n + threads)
+    sc = O.scales(O.LINFREQS, 1000, 1, 500, 20)
+    got = {}
+    for split in ("512", "256"):
+        if split == "256":
+            monkeypatch.setenv("FCWT_HIP_COLS_B256", "1")
+        p = fcwt_amd.Plan(n, sc, 4.0, threads)
+        assert p.row_split()["two_pass"] > 0
+        got[split] = run_device(x, sc, 4.0, threads=threads, plan=p)
+        p.close()
+    assert np.all(np.isfinite(got["512"]))
+    rows = [0, 5, 12, 19]
+    ref = O.cwt_fp64(x, sc, 4.0, threads, True, True, rows=rows)
+    for split, y in got.items():
+        err = O.rel_l2_rows(y[rows], ref)
+        assert err.max() <= TOL, f"{split}: row {rows[int(np.argmax(err))]}: rel-L2 {err.max():.3e}"
+    nz = np.linalg.norm(got["256"], axis=1) > 0
+    assert np.all(got["512"][~nz] == 0)
+    assert O.rel_l2_rows(got["512"][nz], got["256"][nz]).max() <= 2e-6
+
+
+def test_column_path_two_signals_at_2_22():
+    """Two signals of N = 2^22 in one batch plan (the transposed spectra and
+    scratch slots of both, pass 2 writing each signal's rows): each signal's
+    output equals its one-signal plan's bit for bit."""
+    import torch
+
+    lengths = [(1 << 21) + 1, (1 << 22) - 3]
+    xs = [tones_noise(n, seed=k + 30) for k, n in enumerate(lengths)]
+    scs = [O.scales(O.LINFREQS, 1000, 1, 500, 6), O.scales(O.LINFREQS, 1000, 20, 490, 5)]
+    b = fcwt_amd.Batch(lengths, scs, 4.0)
+    assert b.shape()["plans"] == 1
+    dx = [torch.from_numpy(x).cuda() for x in xs]
+    outs = [torch.empty((s.size, x.size), dtype=torch.complex64, device="cuda") for s, x in zip(scs, xs)]
+    b.execute_device([d.data_ptr() for d in dx], [o.data_ptr() for o in outs],
+                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    b.check()
+    for x, sc, o in zip(xs, scs, outs):
+        assert np.array_equal(o.cpu().numpy(), run_device(x, sc, 4.0)), x.size
 
 
 def test_tuning_file_routes_wide_rows(tmp_path):
