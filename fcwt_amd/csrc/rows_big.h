@@ -90,6 +90,19 @@ __device__ __forceinline__ float2 cols_tw(const float2* t1, const float2* t2, in
   return cmul(t2[r >> 7], t1[r & 127]);
 }
 
+// The value that the neighbouring lane (lane l ^ 1) holds in x, by DPP
+// (quad_perm 1, 0, 3, 2); every lane of the wave must be active.
+__device__ __forceinline__ float lane_xor1(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
+// The value that lane Q of this lane's quad holds in x, by DPP (quad_perm
+// Q, Q, Q, Q); every lane of the wave must be active.
+template <int Q>
+__device__ __forceinline__ float lane_quad(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Q * 0x55, 0xF, 0xF, false));
+}
+
 // The windowed band of one pass-1 work-group's columns: column c holds
 // z_c[g] = Xt[qb0 + c][g] psi(qb0 + c + B g) for g < A, 0 beyond.
 struct BandSrc {
@@ -103,7 +116,8 @@ struct BandSrc {
 // points, natural order in and out); Ns = 2^LOG2NS points already combined.
 // IN_G: the pass's input is read from the band instead of LDS: position p of
 // column c after the skipped passes is z_c[p >> log2S] (they only replicate;
-// cols_fft takes this form only when none is skipped).
+// cols_fft takes this form for S = 2^log2S <= 4, where lane pairs or quads
+// share their values and split the window evaluations).
 // OUT_G: the last pass stores its output to the row's scratch U[qb][f]
 // instead of LDS; its lanes write consecutive f.
 template <int LOG2F, int LOG2R, int LOG2NS, bool IN_G, bool OUT_G, int LOG2B>
@@ -127,19 +141,49 @@ __device__ __forceinline__ void cols_pass(float2* d, const float2* t1, const flo
         v[u][k] = gq < src.A ? src.X[(size_t)(src.qb0 + c) * Fh + gq] : make_float2(0.f, 0.f);
       }
     }
+    auto window = [&](int c, int gq) {
+      float w = 0.f;
+      if (gq < src.A) {
+        const int idx = window_index(src.qb0 + c + (gq << LOG2B), src.nseg, src.sseg, src.maxidx);
+        if (idx >= 0) w = mother_value(idx, src.fb, src.toradians, src.mnorm);
+      }
+      return w;
+    };
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int g = threadIdx.x + u * C::T;
       const int c = g >> LOG2GPC, gl = g & ((1 << LOG2GPC) - 1);
+      if constexpr (LOG2NS == 1) {
+        // S = 2: lanes 2i and 2i + 1 hold the same values, so each evaluates
+        // the windows of half the k and takes the other half from its partner
+        const int odd = gl & 1;
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int gq = (gl + (k << LOG2GPC)) >> src.log2S;
-        float w = 0.f;
-        if (gq < src.A) {
-          const int idx = window_index(src.qb0 + c + (gq << LOG2B), src.nseg, src.sseg, src.maxidx);
-          if (idx >= 0) w = mother_value(idx, src.fb, src.toradians, src.mnorm);
+        for (int kk = 0; kk < R / 2; ++kk) {
+          const float wo = window(c, (gl + ((2 * kk + odd) << LOG2GPC)) >> 1);
+          const float wp = lane_xor1(wo);
+          const float w0 = odd ? wp : wo, w1 = odd ? wo : wp;  // k = 2 kk, 2 kk + 1
+          v[u][2 * kk] = make_float2(v[u][2 * kk].x * w0, v[u][2 * kk].y * w0);
+          v[u][2 * kk + 1] = make_float2(v[u][2 * kk + 1].x * w1, v[u][2 * kk + 1].y * w1);
         }
-        v[u][k] = make_float2(v[u][k].x * w, v[u][k].y * w);
+      } else if constexpr (LOG2NS == 2) {
+        // S = 4: the four lanes of a quad hold the same values; lane t
+        // evaluates the windows of k = 4 kk + t and reads the others from
+        // the quad's other lanes
+        const int t = gl & 3;
+#pragma unroll
+        for (int kk = 0; kk < R / 4; ++kk) {
+          const float wo = window(c, (gl + ((4 * kk + t) << LOG2GPC)) >> 2);
+          const float w[4] = {lane_quad<0>(wo), lane_quad<1>(wo), lane_quad<2>(wo), lane_quad<3>(wo)};
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            v[u][4 * kk + q] = make_float2(v[u][4 * kk + q].x * w[q], v[u][4 * kk + q].y * w[q]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          const float w = window(c, (gl + (k << LOG2GPC)) >> src.log2S);
+          v[u][k] = make_float2(v[u][k].x * w, v[u][k].y * w);
+        }
       }
     }
   }
@@ -203,10 +247,11 @@ __device__ __forceinline__ void cols_pass(float2* d, const float2* t1, const flo
 // Pass J of the inverse F-point DFTs of the work-group's columns (radix
 // 2^(LOG2F mod 4) first, radix 16 after) and the passes after it.  Passes
 // before `skip` only replicate the band (cols_skip) and are not run.  With
-// skip = 0 the first pass reads the band itself; otherwise the caller has
-// stored it in LDS, each value at its S places, and pass `skip` reads it
-// there (reading it from the band would evaluate each window S times).  The
-// last pass stores to U; the passes between exchange through LDS.
+// S <= 4 (the radix product of the skipped passes) pass `skip` reads the
+// band itself; otherwise the caller has stored it in LDS, each value at its
+// S places, and pass `skip` reads it there (reading it from the band would
+// evaluate each window S times).  The last pass stores to U; the passes
+// between exchange through LDS.
 template <int LOG2F, int LOG2B, int J>
 __device__ __forceinline__ void cols_fft(float2* d, const float2* t1, const float2* t2, int skip,
                                          const BandSrc& src, gfloat2* U) {
@@ -217,8 +262,11 @@ __device__ __forceinline__ void cols_fft(float2* d, const float2* t1, const floa
     constexpr int LOG2R = (J == 0 && FIRST > 0) ? FIRST : 4;
     constexpr int LOG2NS = J == 0 ? 0 : (FIRST > 0 ? FIRST + 4 * (J - 1) : 4 * J);
     constexpr bool LAST = J == NPASS - 1;
-    if constexpr (J == 0) {
-      if (skip == 0) cols_pass<LOG2F, LOG2R, LOG2NS, true, LAST, LOG2B>(d, t1, t2, src, U);
+    if constexpr (LOG2NS <= 2) {
+      if (skip == J)
+        cols_pass<LOG2F, LOG2R, LOG2NS, true, LAST, LOG2B>(d, t1, t2, src, U);
+      else if (skip < J)
+        cols_pass<LOG2F, LOG2R, LOG2NS, false, LAST, LOG2B>(d, t1, t2, src, U);
     } else if (skip <= J) {
       cols_pass<LOG2F, LOG2R, LOG2NS, false, LAST, LOG2B>(d, t1, t2, src, U);
     }
@@ -304,7 +352,7 @@ __device__ __forceinline__ void cols_pass1(const BigArgs& a, int y, int xblk, fl
   constexpr int FIRST = LOG2F % 4;
   constexpr int NPASS = (FIRST > 0) + (LOG2F - FIRST) / 4;
   if (skip < NPASS) {
-    if (skip > 0) {
+    if (src.log2S >= 3) {
       // the columns' windowed bands into LDS, each value at its S places;
       // all loads are issued before the first is used
       const int log2G = LOG2F - src.log2S;  // values per column
