@@ -419,7 +419,7 @@ TRACES = [
      "Calls include the 50 warm-up steps, the 100 timed steps and the 10 steps of the per-kernel event pass of "
      "bench.py."),
     ("trace_long", "cwt_long_4194304_x_1024", "long signal, n = 2^22 x 1024 LINFREQS (sigma 4)",
-     "--workload long --steps 3 --warmup 2",
+     "--workload long --steps 3 --warmup 2 --no-tune",
      "Calls include the 2 warm-up steps, the 3 timed steps and the profiled step of bench.py."),
     ("trace_batched", "cwt_batched_384_channels", "384 channels x 200 LOGSCALES (sigma 6)",
      "--workload batched --steps 3 --warmup 2",

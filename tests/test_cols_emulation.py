@@ -1,11 +1,14 @@
-"""CPU emulation of the column path's pass-1 index arithmetic
+"""CPU emulation of the column path's index arithmetic. Pass 1
 (fcwt_amd/csrc/rows_big.h: cols_skip, cols_fft, cols_pass): for every F of
 the path and bands A <= F/2, the passes that only replicate are skipped, the
 first pass run reads position p of a column as z[p >> log2S], the passes in
 between are radix-2^(log2F mod 4) then radix-16 Stockham steps with the
 kernel's twiddle exponents, and the last pass writes position
 dst + k Ns.  The result must be the inverse DFT of the band,
-sum_{q < A} z[q] e^{+2 pi i q f / F}, to float64 rounding.  No GPU."""
+sum_{q < A} z[q] e^{+2 pi i q f / F}, to float64 rounding.  Pass 2 at
+B = 512 (cols_pass2_b512): the two register stages, the LDS exchange
+indices and the lane-pair radix-2 step, against the direct sum over qb.
+No GPU."""
 import numpy as np
 import pytest
 
@@ -71,3 +74,52 @@ def test_pass1_transform_every_F(log2F):
         ref = np.fft.ifft(np.concatenate([z, np.zeros(F - A)])) * F
         err = np.abs(got - ref).max() / np.abs(ref).max()
         assert err < 1e-12, (log2F, A, err)
+
+
+def test_pass2_512_point_decomposition():
+    """cols_pass2_b512 (rows_big.h): 512 threads per 16 f, qb = j + 32 k,
+    m = m1 + 16 m2.  Stage 1 (thread g = tid & 15, j = tid >> 4): W_N^{32 k f}
+    from the [k][g] table, a 16-point DFT over k, then c W_N^{j f} W_512^{j m1}
+    (W_512^{j m1} = W_N^{(j m1) << log2F}, the [m1][j] table); the exchange
+    xb[m1][g][j]; stage 2 (g, e = (tid >> 4) & 1, m1 = tid >> 5): a 16-point
+    DFT over j' of xb[m1][g][2 j' + e]; then lane e = 1 multiplies by
+    W_32^{m2'} (the table entry [m2'][16]) and the lanes 16 apart swap:
+    e = 0 writes E_0 + W E_1 at m = m1 + 16 m2', e = 1 writes E_0 - W E_1 at
+    m + 256.  Against y[f + F m] = sum_qb W_512^{qb m} W_N^{qb f} U[qb][f]."""
+    rng = np.random.default_rng(5)
+    log2F = 6
+    F, B = 1 << log2F, 512
+    N = B * F
+    U = rng.standard_normal((B, F)) + 1j * rng.standard_normal((B, F))
+
+    def W(X, r):
+        return np.exp(2j * np.pi * np.asarray(r) / X)
+
+    ref = np.zeros(N, complex)
+    for f in range(F):
+        ref[f + F * np.arange(B)] = np.fft.ifft(U[:, f] * W(N, np.arange(B) * f)) * B
+    wb = W(N, (np.arange(16)[:, None] * np.arange(32)[None, :]) << log2F)  # [m1][j]
+    got = np.zeros(N, complex)
+    k = np.arange(16)
+    for f0 in range(0, F, 16):
+        xb = np.zeros((16, 16, 33), complex)
+        for tid in range(512):
+            g, j = tid & 15, tid >> 4
+            f = f0 + g
+            v = U[j + 32 * k, f] * W(N, (32 * k * f) % N)
+            v = np.fft.ifft(v) * 16
+            v = v * W(N, j * f) * wb[:, j]
+            xb[:, g, j] = v
+        E = np.zeros((512, 16), complex)
+        for tid in range(512):
+            g, e, m1 = tid & 15, (tid >> 4) & 1, tid >> 5
+            E[tid] = np.fft.ifft(xb[m1, g, 2 * np.arange(16) + e]) * 16
+        x = E.copy()
+        odd = ((np.arange(512) >> 4) & 1) == 1
+        x[odd] *= wb[np.arange(16), 16][None, :]
+        for tid in range(512):
+            g, e, m1 = tid & 15, (tid >> 4) & 1, tid >> 5
+            pr = x[tid ^ 16]
+            o = pr - x[tid] if e else x[tid] + pr
+            got[f0 + g + F * (m1 + 16 * np.arange(16) + 256 * e)] = o
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
