@@ -73,6 +73,7 @@ SIGNATURES = {
                                                      ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p]),
     "fcwt_hip_batch_check": (ctypes.c_int, [ctypes.c_void_p]),
     "fcwt_hip_batch_shape": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p]),
+    "fcwt_hip_batch_row_launches": (ctypes.c_int, [ctypes.c_void_p, _c_int_p]),
     "fcwt_hip_batch_graph_launches": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
 }
 

@@ -277,9 +277,12 @@ class Batch:
         self._outs = (ctypes.c_void_p * cnt)()
 
     def shape(self) -> dict:
-        g, t = ctypes.c_int(), ctypes.c_int()
+        """Plans (padded sizes), rows on the XCD-team kernel, and row-kernel
+        launches per execute."""
+        g, t, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         nat.check(nat.lib().fcwt_hip_batch_shape(self._h, ctypes.byref(g), ctypes.byref(t)), "fcwt_hip_batch_shape")
-        return {"plans": g.value, "team_rows": t.value}
+        nat.check(nat.lib().fcwt_hip_batch_row_launches(self._h, ctypes.byref(r)), "fcwt_hip_batch_row_launches")
+        return {"plans": g.value, "team_rows": t.value, "row_launches": r.value}
 
     def execute_device(self, x_ptrs, out_ptrs, stream: int = 0) -> None:
         """x_ptrs[c]: lengths[c] float32 on the device; out_ptrs[c]: fns[c] x

@@ -140,6 +140,12 @@ static int direct_phases(int log2M) {
 // the same for a block run by the team kernel (kFillMinLog2M <= log2M <= 10)
 static int direct_phases_team(int log2M) {
   switch (log2M) {
+    case 0: return DirectCfg<0, true>::G;
+    case 1: return DirectCfg<1, true>::G;
+    case 2: return DirectCfg<2, true>::G;
+    case 3: return DirectCfg<3, true>::G;
+    case 4: return DirectCfg<4, true>::G;
+    case 5: return DirectCfg<5, true>::G;
     case 6: return DirectCfg<6, true>::G;
     case 7: return DirectCfg<7, true>::G;
     case 8: return DirectCfg<8, true>::G;
@@ -924,6 +930,16 @@ static bool fill_rows(size_t team_rows, int N, int signals) {
   return team_rows > 0 && (signals > 1 || (double)team_rows * (double)N <= kFillMaxTeamPoints);
 }
 
+// Scratch buffers per team by default: as many N-point buffers as fit
+// kTeamScratchPerXcd (6 MiB), within [kTeamMinBuffers, kTeamBuffers].  At
+// N = 2^18 that is 6 buffers of 2 MiB (12 MiB per team, most of it in the
+// memory-side cache): fewer leave the sequence too little slack.
+constexpr int kTeamMinBuffers = 6;
+static int team_default_buffers(int N) {
+  const size_t row_bytes = sizeof(float2) * (size_t)N;
+  return (int)std::max<size_t>(kTeamMinBuffers, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+}
+
 static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host) {
   const bool avx = !(p->flags & FCWT_HIP_COVERAGE_SCALAR);
   // each signal's rows, with that signal's reuse of stale bins (one
@@ -936,11 +952,12 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
   }
   p->rows.resize(p->fn);
   std::vector<int> wide, team, cols;
-  const int team_log2F = p->log2N - kTeamLog2B;
+  const int team_log2B = team_log2b(p->log2N), team_log2F = p->log2N - team_log2B;
   const Tuned tn = tuned(p->N);
-  const bool team_ok =
-      p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F && tn.path != 2;
-  const bool cols_ok = team_log2F >= kColsMinLog2F && team_log2F <= kColsMaxLog2F && !cols_disabled();
+  const bool team_ok = p->team_capable && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F &&
+                       p->log2N <= kTeamMaxLog2N && tn.path != 2;
+  const int log2F256 = p->log2N - kTeamLog2B;  // the column path's F at B = 256
+  const bool cols_ok = log2F256 >= kColsMinLog2F && log2F256 <= kColsMaxLog2F && !cols_disabled();
   const int cols_lb = cols_split(p->log2N, tn), cols_lf = p->log2N - cols_lb;
   for (int i = 0, sgi = 0; i < p->fn; ++i) {
     while (i >= p->sig_row0[sgi] + p->sig_fn[sgi]) ++sgi;
@@ -954,9 +971,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     log2M = std::min(log2M, p->log2N);
     // the tuned crossover moves M = 1024 bands to the team kernel (A = 4)
     const int direct_max = tn.direct_max == 9 ? 9 : kMaxLog2B;
-    if (log2M > direct_max && log2M > kTeamLog2B && team_ok && r.seg_count <= kTeamSegLds) {
-      r.log2B = kTeamLog2B;
-      r.log2A = log2M - kTeamLog2B;
+    if (log2M > direct_max && log2M > team_log2B && team_ok && r.seg_count <= kTeamSegLds) {
+      r.log2B = team_log2B;
+      r.log2A = log2M - team_log2B;
       team.push_back(i);
     } else if (log2M > kMaxLog2B && cols_ok && r.seg_count <= kTeamSegLds) {
       r.log2B = cols_lb;
@@ -978,10 +995,9 @@ static int build_schedule(fcwt_hip_plan p, const std::vector<float>& mother_host
     p->nteams = 8;
     p->team_log2F = team_log2F;
     // scratch buffers per team: as many N-point buffers as fit the per-team
-    // scratch budget (the XCD's L2 holds them), unless tuned
+    // scratch budget, and at least kTeamMinBuffers, unless tuned
     // (fcwt_hip_tune_load); FCWT_HIP_TEAM_BUFFERS overrides both
-    const size_t row_bytes = sizeof(float2) * (size_t)p->N;
-    p->team_nbuf = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+    p->team_nbuf = team_default_buffers(p->N);
     if (tn.team_buffers) p->team_nbuf = tn.team_buffers;
     if (const char* e = std::getenv("FCWT_HIP_TEAM_BUFFERS")) {
       int v = std::atoi(e);
@@ -1127,11 +1143,12 @@ static int check_device(int* dev) {
 // launch) and makes one team of the work-groups each XCD receives, whose
 // scratch stays in that XCD's L2: it is meant for the full 256-CU device in
 // one partition.  FCWT_HIP_NO_TEAMS=1 in the environment turns it off.
-static const void* team_kernel(int log2F) {
-  switch (log2F) {
-    case 7: return (const void*)k_rows_team<7>;
-    case 8: return (const void*)k_rows_team<8>;
-    case 9: return (const void*)k_rows_team<9>;
+static const void* team_kernel(int log2N) {
+  switch (log2N) {
+    case 15: return (const void*)k_rows_team<7, 8>;
+    case 16: return (const void*)k_rows_team<8, 8>;
+    case 17: return (const void*)k_rows_team<9, 8>;
+    case 18: return (const void*)k_rows_team<9, 9>;
     default: return nullptr;
   }
 }
@@ -1139,7 +1156,7 @@ static const void* team_kernel(int log2F) {
 static bool team_capable(int dev, int log2N) {
   const char* env = std::getenv("FCWT_HIP_NO_TEAMS");
   if (env && env[0] == '1') return false;
-  const void* kern = team_kernel(log2N - kTeamLog2B);
+  const void* kern = team_kernel(log2N);
   if (!kern) return false;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return false;
@@ -1441,8 +1458,31 @@ static long team_spin() {
   return kTeamSpin;
 }
 
+// The team kernel's arguments for plan p (its sync words at `sync`).
+static TeamArgs team_args_of(fcwt_hip_plan p, const KArgs& a, unsigned* sync) {
+  TeamArgs ta;
+  ta.k = a;
+  ta.team_desc = p->d_team_desc;
+  ta.team_blocks = p->d_team_blocks;
+  ta.seq_stride = p->seq_stride;
+  ta.nteams = p->nteams;
+  ta.log2F = p->team_log2F;
+  ta.nbuf = p->team_nbuf;
+  ta.spin = team_spin();
+  ta.scratch = p->d_team_scratch;
+  ta.sync = sync;
+  ta.error = p->d_error;
+  ta.fill_items = p->d_items + p->fill_begin;
+  ta.nfill = (int)p->fill_count;
+  for (int g = 0; g < kFillGroups; ++g) ta.fill_end[g] = p->fill_end[g];
+  return ta;
+}
+
+// Every launch of one execute of plan p on st.  With `defer_team` non-null
+// the team launch is not made: its arguments go there, for a launch that
+// takes the team rows of several plans (k_rows_team_multi).
 static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, float2* const* outs,
-                      hipStream_t st) {
+                      hipStream_t st, TeamArgs* defer_team = nullptr) {
   const bool multi = p->nsig > 1;
   if (multi) {
     int rc = upload_pointers(p, xs, outs, st);
@@ -1469,27 +1509,15 @@ static int launch_all(fcwt_hip_plan p, const float* const* xs, bool complex_in, 
       launch_direct(gr.log2M, (unsigned)gr.count, a, st);
     }
   }
-  if (p->nteams) {
+  if (p->nteams && defer_team) {
+    *defer_team = team_args_of(p, a, nullptr);
+  } else if (p->nteams) {
     ProfScope ps(p, 4, st);
     size_t words = p->team_sync_words;
     k_fill_u32<<<(unsigned)((words + 255) / 256), 256, 0, st>>>(p->d_team_sync, words, 0u);
-    TeamArgs ta;
-    ta.k = a;
-    ta.team_desc = p->d_team_desc;
-    ta.team_blocks = p->d_team_blocks;
-    ta.seq_stride = p->seq_stride;
-    ta.nteams = p->nteams;
-    ta.log2F = p->team_log2F;
-    ta.nbuf = p->team_nbuf;
-    ta.spin = team_spin();
-    ta.scratch = p->d_team_scratch;
-    ta.sync = p->d_team_sync;
-    ta.error = p->d_error;
-    ta.fill_items = p->d_items + p->fill_begin;
-    ta.nfill = (int)p->fill_count;
-    for (int g = 0; g < kFillGroups; ++g) ta.fill_end[g] = p->fill_end[g];
+    TeamArgs ta = team_args_of(p, a, p->d_team_sync);
     void* args[] = {&ta};
-    HIP_TRY(hipLaunchKernel(team_kernel(p->team_log2F), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0, st));
+    HIP_TRY(hipLaunchKernel(team_kernel(p->log2N), dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0, st));
   }
   if (!p->cols_batches.empty()) {
     const int lf = p->cols_log2F, lb = p->cols_log2B;
@@ -1817,7 +1845,7 @@ int fcwt_hip_plan_row_split(fcwt_hip_plan p, int* direct_rows, int* team_rows, i
   int d = 0, t = 0, w = 0;
   for (const RowD& r : p->rows) {
     if (r.log2A == 0) ++d;
-    else if (r.log2B == kTeamLog2B && r.slot < 0) ++t;
+    else if (r.slot < 0) ++t;  // team rows hold no scratch slot
     else ++w;
   }
   if (direct_rows) *direct_rows = d;
@@ -2100,7 +2128,56 @@ struct fcwt_hip_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   long long graph_launches = 0;
+  // One team launch for the team rows of every group (k_rows_team_multi),
+  // when two or more groups have them: its control words, then each such
+  // group's sync words (at msync_off[group]), all zeroed by one fill kernel
+  // before the launch.
+  bool team_multi = false;
+  unsigned* d_msync = nullptr;
+  size_t msync_words = 0;
+  std::vector<size_t> msync_off;
 };
+
+// Whether a batch runs the team rows of all its padded sizes in one launch
+// (k_rows_team_multi) rather than one launch per size.
+// FCWT_HIP_BATCH_TEAM_LAUNCHES=separate keeps one launch per size.
+static bool batch_team_multi() {
+  const char* e = std::getenv("FCWT_HIP_BATCH_TEAM_LAUNCHES");
+  return !(e && std::strcmp(e, "separate") == 0);
+}
+
+// Every launch of one execute of batch b on st: each group's forward
+// transforms and row launches, and (team_multi) one team launch for all of
+// them after the last group's other launches.
+static int launch_batch(fcwt_hip_batch b, const std::vector<std::vector<const float*>>& xs,
+                        const std::vector<std::vector<float2*>>& outs, hipStream_t st) {
+  TeamMulti m;
+  std::memset(&m, 0, sizeof(m));
+  fcwt_hip_plan prof = nullptr;  // the plan whose profile counts the team launch
+  for (size_t k = 0; k < b->groups.size(); ++k) {
+    fcwt_hip_plan p = b->groups[k].plan;
+    const bool defer = b->team_multi && p->nteams > 0;
+    TeamArgs ta;
+    int rc = launch_all(p, xs[k].data(), false, outs[k].data(), st, defer ? &ta : nullptr);
+    if (rc != FCWT_HIP_OK) return rc;
+    if (defer) {
+      const int s = kTeamMaxLog2N - p->log2N;  // seg[0] holds N = 2^18
+      ta.sync = b->d_msync + b->msync_off[k];
+      m.seg[s] = ta;
+      m.used |= 1 << s;
+      if (!prof) prof = p;
+    }
+  }
+  if (m.used) {
+    ProfScope ps(prof, 4, st);
+    m.ctl = b->d_msync;
+    k_fill_u32<<<(unsigned)((b->msync_words + 255) / 256), 256, 0, st>>>(b->d_msync, b->msync_words, 0u);
+    void* args[] = {&m};
+    HIP_TRY(hipLaunchKernel((const void*)k_rows_team_multi, dim3(256 * kTeamWgPerCu), dim3(kTeamThreads), args, 0,
+                            st));
+  }
+  return FCWT_HIP_OK;
+}
 
 static void drop_graph(fcwt_hip_batch b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
@@ -2147,6 +2224,25 @@ int fcwt_hip_batch_create(fcwt_hip_batch* out, int count, const int* lengths, co
     b->groups.push_back(std::move(g));
     k = e;
   }
+  int team_groups = 0;
+  for (auto& g : b->groups) team_groups += g.plan->nteams > 0;
+  if (team_groups >= 2 && batch_team_multi()) {
+    b->msync_words = 64;  // the launch's abort and exit words
+    b->msync_off.assign(b->groups.size(), 0);
+    for (size_t k = 0; k < b->groups.size(); ++k)
+      if (b->groups[k].plan->nteams > 0) {
+        b->msync_off[k] = b->msync_words;
+        b->msync_words += b->groups[k].plan->team_sync_words;
+      }
+    hipError_t e = hipMalloc((void**)&b->d_msync, b->msync_words * sizeof(unsigned));
+    if (e != hipSuccess) {
+      for (auto& h : b->groups) free_plan(h.plan);
+      delete b;
+      return fail(FCWT_HIP_ENOMEM, std::string("fcwt_hip_batch_create: hipMalloc of the team launch's words: ") +
+                                       hipGetErrorString(e));
+    }
+    b->team_multi = true;
+  }
   *out = b;
   return FCWT_HIP_OK;
 }
@@ -2155,6 +2251,7 @@ int fcwt_hip_batch_destroy(fcwt_hip_batch b) {
   if (!b) return FCWT_HIP_OK;
   drop_graph(b);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  if (b->d_msync) (void)hipFree(b->d_msync);
   for (auto& g : b->groups) free_plan(g.plan);
   delete b;
   return FCWT_HIP_OK;
@@ -2208,9 +2305,7 @@ int fcwt_hip_batch_execute_device(fcwt_hip_batch b, const float* const* d_inputs
     bool ok = b->cap || hipStreamCreate(&b->cap) == hipSuccess;
     ok = ok && hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-      int lrc = FCWT_HIP_OK;
-      for (size_t k = 0; k < b->groups.size() && lrc == FCWT_HIP_OK; ++k)
-        lrc = launch_all(b->groups[k].plan, xs[k].data(), false, outs[k].data(), b->cap);
+      const int lrc = launch_batch(b, xs, outs, b->cap);
       hipGraph_t g = nullptr;
       ok = hipStreamEndCapture(b->cap, &g) == hipSuccess && lrc == FCWT_HIP_OK && g;
       if (ok) {
@@ -2231,8 +2326,7 @@ int fcwt_hip_batch_execute_device(fcwt_hip_batch b, const float* const* d_inputs
     b->graphs = false;
     (void)hipGetLastError();
   }
-  for (size_t k = 0; k < b->groups.size() && rc == FCWT_HIP_OK; ++k)
-    rc = launch_all(b->groups[k].plan, xs[k].data(), false, outs[k].data(), st);
+  rc = launch_batch(b, xs, outs, st);
   if (cur != dev) (void)hipSetDevice(cur);
   return rc;
 }
@@ -2245,11 +2339,18 @@ int fcwt_hip_batch_graph_launches(fcwt_hip_batch b, long long* launches) {
 
 int fcwt_hip_batch_check(fcwt_hip_batch b) {
   if (!b) return fail(FCWT_HIP_EINVAL, "null batch");
+  // every plan's report is read (and cleared); the first failure is returned
+  int first = FCWT_HIP_OK;
+  std::string msg;
   for (auto& g : b->groups) {
-    int rc = fcwt_hip_plan_check(g.plan);
-    if (rc != FCWT_HIP_OK) return rc;
+    const int rc = fcwt_hip_plan_check(g.plan);
+    if (rc != FCWT_HIP_OK && first == FCWT_HIP_OK) {
+      first = rc;
+      msg = g_last_error;
+    }
   }
-  return FCWT_HIP_OK;
+  if (first != FCWT_HIP_OK) g_last_error = msg;
+  return first;
 }
 
 // ------------------------------------------------------------- tuning
@@ -2300,20 +2401,21 @@ int fcwt_hip_tune(int N, int threads, int effort, const char* path) {
   if (N < 2 || (N & (N - 1)) || threads < 1 || effort < 0 || effort > 2 || !path)
     return fail(FCWT_HIP_EINVAL, "fcwt_hip_tune: bad arguments");
   const int log2N = find2power(N);
-  const int log2F = log2N - kTeamLog2B;
+  const int log2F = log2N - kTeamLog2B;  // at B = 256 (the column path's split below 2^22)
+  const int team_log2F = log2N - team_log2b(log2N);
   int rc = FCWT_HIP_OK;
   bool teams = false;
   if (effort > 0) {  // effort 0 writes the defaults and needs no device
     int dev = 0;
     rc = check_device(&dev);
     if (rc != FCWT_HIP_OK) return rc;
-    teams = team_capable(dev, log2N) && log2F >= kTeamMinLog2F && log2F <= kTeamMaxLog2F;
+    teams = team_capable(dev, log2N) && team_log2F >= kTeamMinLog2F && team_log2F <= kTeamMaxLog2F &&
+            log2N <= kTeamMaxLog2N;
   }
   const bool cols = log2F >= kColsMinLog2F && log2F <= kColsMaxLog2F;
-  const size_t row_bytes = sizeof(float2) * (size_t)N;
   // the built-in choice, written as is for effort 0 or sizes with nothing to choose
   Tuned best;
-  best.team_buffers = (int)std::max<size_t>(2, std::min<size_t>(kTeamBuffers, kTeamScratchPerXcd / row_bytes));
+  best.team_buffers = team_default_buffers(N);
   best.cols_mb = (int)(kColsScratchBytes >> 20);
   best.path = 0;  // the built-in routing unless a timed candidate wins
   float best_ms = -1.f, default_ms = -1.f;
@@ -2412,6 +2514,32 @@ int fcwt_hip_tune_load(const char* path) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[N] = t;
   ++g_tune_generation;
+  return FCWT_HIP_OK;
+}
+
+// Row-kernel launches one execute of plan p makes, its team launch counted
+// only when `team` (launch_all).
+static int plan_row_launches(const fcwt_hip_plan p, bool team) {
+  int n = 0;
+  for (const auto& gr : p->direct_groups) n += !gr.fill;
+  if (p->nteams && team) ++n;
+  if (!p->cols_batches.empty()) {
+    const int nb = (int)p->cols_batches.size();
+    n += 1 + (p->cols_sets == 2 ? nb + 1 : 2 * nb);  // the spectrum transpose, then the passes
+  }
+  n += 2 * (int)p->batches.size();
+  return n;
+}
+
+int fcwt_hip_batch_row_launches(fcwt_hip_batch b, int* launches) {
+  if (!b || !launches) return fail(FCWT_HIP_EINVAL, "fcwt_hip_batch_row_launches: null argument");
+  int n = 0;
+  bool any_team = false;
+  for (auto& g : b->groups) {
+    n += plan_row_launches(g.plan, !b->team_multi);
+    any_team |= g.plan->nteams > 0;
+  }
+  *launches = n + (b->team_multi && any_team ? 1 : 0);
   return FCWT_HIP_OK;
 }
 

@@ -458,16 +458,6 @@ __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, in
   }
 }
 
-// The value that lane l ^ 16 holds in x (lanes 16 apart, within each
-// 32-lane half of the wave), from v_permlane16_swap: that instruction swaps
-// the odd 16-lane rows of its first operand with the even rows of its
-// second, so with x in both, the first result holds the partner's value in
-// the odd rows and the second in the even rows.
-__device__ __forceinline__ float lane_xor16(float x, bool odd_row) {
-  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-  return __uint_as_float(odd_row ? r[0] : r[1]);
-}
-
 // Pass 2 at B = 512 for row y of the batch, f block fblk (16 consecutive f),
 // by the work-group's 512 threads.  With qb = j + 32 k and m = m1 + 16 m2:
 //   y[f + F m] = c sum_j W_32^{j m2} W_512^{j m1} W_N^{j f}

@@ -27,16 +27,21 @@
 namespace fcwt_hip {
 
 // TEAM: the configuration of a block run inside the wide-band team kernel
-// (rows_team.h), whose work-groups have 256 threads: at M = 1024 a block
-// takes 8 phases there (64-byte output runs) instead of 16.
+// (rows_team.h), whose work-groups have 256 threads and 4768 complex64 values
+// of LDS for a block: at M = 1024 a block takes 8 phases there (64-byte
+// output runs) instead of 16, and at M = 16 and 32 (one thread per phase)
+// 128 and 64 phases, so that the output transpose fits, with the other
+// threads idle until the stores.
 template <int LOG2M, bool TEAM = false>
 struct DirectCfg {
   static constexpr int M = 1 << LOG2M;
   static constexpr int R1 = M < 32 ? M : 32;
   static constexpr int P2 = M / R1;
   // phases per work-group (upper bound; fewer when the row has fewer phases)
-  static constexpr int G = P2 >= 16 ? (TEAM && P2 > 16 ? 256 / P2 : 16) : 256 / P2;
-  static constexpr int NT = G * P2;                // threads
+  static constexpr int G = P2 >= 16   ? (TEAM && P2 > 16 ? 256 / P2 : 16)
+                           : P2 > 1 || !TEAM ? 256 / P2
+                                             : (M >= 32 ? 64 : M >= 16 ? 128 : 256);
+  static constexpr int NT = TEAM ? 256 : G * P2;  // threads
   static constexpr int XS = M + P2;                // LDS stride between phases in the exchange
   static constexpr int OS = G + 1;                 // LDS stride between m rows in the transpose
   // Tables: Z[0, M), W_M^{j k1}, the pre-twiddles.  With P2 > 1 the exchange

@@ -3,14 +3,16 @@
 // one XCD exchanging the intermediate through that XCD's L2.
 // Device code, included by fcwt_hip.hip.
 //
-// With N = B * F (B = 256) and the band M = A * B:
+// With N = B * F (B = 256 for N = 2^15 .. 2^17, B = 512 and F = 512 for
+// N = 2^18) and the band M = A * B:
 //   y[f + F*m] = c * sum_{qb<B} W_B^{qb m} W_N^{qb f} U[f][qb]
 //   U[f][qb]   = sum_{qf<A} Z[qb + B qf] W_F^{qf f}
 // A pass-1 item (Qb = 4096/F consecutive qb): the F-point sums, done as F/A
 // phases l' of A-point inverse FFTs (f = l' + (F/A) m'), into a scratch
-// buffer U[f][qb] in global memory.  A pass-2 item (16 consecutive f): the
-// factor W_N^{qb f} and the B-point inverse FFTs over qb, written as
-// 16-element (128-byte) runs of the output row.  A row is F/16 items of each.
+// buffer U[f][qb] in global memory.  A pass-2 item (4096/B consecutive f):
+// the factor W_N^{qb f} and the B-point inverse FFTs over qb, written as
+// runs of 16 (B = 256, 128 bytes) or 8 (B = 512, 64 bytes) elements of the
+// output row.  A row is N/4096 items of each pass.
 //
 // One team per XCD: all work-groups the hardware places on that XCD (found
 // from the XCC id), so the scratch stays in the XCD's L2.  The team walks a
@@ -55,19 +57,23 @@ constexpr int kTeamWgPerXcd = 32 * kTeamWgPerCu;
 constexpr int kTeamPoints = 4096;              // points per item
 constexpr int kTeamBuffers = 12;               // most scratch buffers per team
 constexpr size_t kTeamScratchPerXcd = 6u << 20;  // scratch bytes per team (default buffer count)
-constexpr int kTeamMaxLog2F = 9;               // F <= 512, N <= 2^17
+constexpr int kTeamMaxLog2F = 9;               // F <= 512
 constexpr int kTeamMinLog2F = 7;               // F >= 128, N >= 2^15
+constexpr int kTeamMaxLog2N = 18;              // N = 2^18 splits as B = 512, F = 512
+// log2 B of the team split of N = 2^log2N: 256 up to 2^17, 512 at 2^18 (so
+// that F, and with it pass 1's tables and tile, stays within 512)
+__host__ __device__ constexpr int team_log2b(int log2N) { return log2N > 17 ? 9 : kTeamLog2B; }
 constexpr int kTeamSegLds = 32;                // most band segments of a row on the team and column paths
 constexpr long kTeamSpin = 1L << 22;           // default polls (with s_sleep) before a wait is declared failed
-constexpr int kFillMinLog2M = 6;               // smallest band of a direct row the team kernel runs
-constexpr int kFillGroups = 10 - kFillMinLog2M + 1;  // bands 2^6 .. 2^10
+constexpr int kFillMinLog2M = 0;               // smallest band of a direct row the team kernel runs
+constexpr int kFillGroups = 10 - kFillMinLog2M + 1;  // bands 2^0 .. 2^10
 
 // One block of a team's item sequence with what its items read about their
 // row, so that an item starts after one burst of scalar loads.
 struct TeamBlk {
   int sq;                    // 2 i + pass: row i of the team, pass 0 or 1
   int seg_begin, seg_count;  // the row's band segments in KArgs::segs
-  int log2A;                 // band M = 256 A
+  int log2A;                 // band M = B A
   int sig, n;                // signal of the row and its length
   long long out_off;         // offset of the row in its signal's output
   SegD seg[2];               // the first two segments (any further ones stay in segs)
@@ -80,7 +86,7 @@ struct TeamArgs {
   const int* team_blocks;  // blocks in team t's sequence
   int seq_stride;
   int nteams;              // teams on the whole device (one per XCD)
-  int log2F;               // F = N / 256 (the kernel's template parameter)
+  int log2F;               // F = N / B (the kernel's template parameter)
   int nbuf;                // scratch buffers per team (2..kTeamBuffers)
   long spin;               // polls before a wait is declared failed (kTeamSpin unless a test lowers it)
   float2* scratch;         // [team][nbuf][N]
@@ -170,32 +176,40 @@ __device__ __forceinline__ bool team_wait(const unsigned* p, unsigned target, un
   return *s_flag != 0;
 }
 
-// A work-group's exit.  Its stores are complete before it counts itself
-// out, so the last work-group to finish knows that no other writes any more.
-// It checks that every team claimed its whole sequence, and when the launch
-// has aborted (or a team's sequence was left unclaimed) it fills every team
-// row of the launch with NaN: a failed launch never leaves rows that look
-// valid.
-__device__ __noinline__ void team_exit(unsigned* sync, unsigned* error, const TeamBlk* team_desc,
-                                      const int* team_blocks, int seq_stride, int nteams, int log2I, float2* out1,
-                                      float2* const* outs, int* s_flag) {
+// A work-group's exit.  Its stores are complete before it counts itself out
+// of the launch (ctl[kSyncExited]), so the last work-group to finish knows
+// that no other writes any more: true, for every thread, in that one.  It
+// then checks that every team claimed its whole sequence
+// (team_check_claims), and when the launch has aborted (or a team's sequence
+// was left unclaimed) it fills every team row of the launch with NaN
+// (team_nan_rows): a failed launch never leaves rows that look valid.
+__device__ __noinline__ bool team_exit_last(unsigned* ctl, int* s_flag) {
   wait_stores();
   __syncthreads();
   if (threadIdx.x == 0) {
     __atomic_thread_fence(__ATOMIC_RELEASE);
-    const unsigned done = __hip_atomic_fetch_add(sync + kSyncExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    bool fill = false;
-    if (done + 1 == gridDim.x) {
-      for (int t = 0; t < nteams && ld_agent(sync + kSyncAbort) == 0; ++t) {
-        const unsigned claimed = ld_agent(sync + 64 * (1 + nteams * kTeamBuffers + t));
-        if (claimed < ((unsigned)team_blocks[t] << log2I)) team_fail(sync + kSyncAbort, error, 2u);
-      }
-      fill = ld_agent(sync + kSyncAbort) != 0;
-    }
-    *s_flag = fill;
+    const unsigned done = __hip_atomic_fetch_add(ctl + kSyncExited, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    *s_flag = done + 1 == gridDim.x;
   }
   __syncthreads();
-  if (!*s_flag) return;
+  return *s_flag != 0;
+}
+
+// Thread 0 of the last work-group: every team of one plan's sequences was
+// claimed to its end (an XCD that received no work-group would leave its
+// rows unwritten); a shortfall aborts the launch with code 2.
+__device__ __noinline__ void team_check_claims(const unsigned* sync, unsigned* abort, unsigned* error,
+                                               const int* team_blocks, int nteams, int log2I) {
+  for (int t = 0; t < nteams && ld_agent(abort) == 0; ++t) {
+    const unsigned claimed = ld_agent(sync + 64 * (1 + nteams * kTeamBuffers + t));
+    if (claimed < ((unsigned)team_blocks[t] << log2I)) team_fail(abort, error, 2u);
+  }
+}
+
+// Every thread of the last work-group of an aborted launch: one plan's team
+// rows filled with NaN.
+__device__ __noinline__ void team_nan_rows(const TeamBlk* team_desc, const int* team_blocks, int seq_stride,
+                                           int nteams, float2* out1, float2* const* outs) {
   const float qnan = __int_as_float(0x7fc00000);
   for (int t = 0; t < nteams; ++t) {
     const int nb = team_blocks[t];
@@ -289,16 +303,27 @@ __shared__ float2 tm_buf[kTeamBuf];
 __shared__ float2 tm_pre[1 << kTeamMaxLog2F];   // pass-1 [k][l']
 __shared__ float2 tm_post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
 __shared__ float2 tm_t2[kTeamB];                // pass-2 [g][k]
-__shared__ float2 tm_wb[kTeamB];                // pass-2 [k1][j]
+__shared__ float2 tm_wb[512];                   // pass-2 [k1][j] (B = 256) or [m1][j] (B = 512)
 
-// Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A.
+// The value that lane l ^ 16 holds in x (lanes 16 apart, within each
+// 32-lane half of the wave), from v_permlane16_swap: that instruction swaps
+// the odd 16-lane rows of its first operand with the even rows of its
+// second, so with x in both, the first result holds the partner's value in
+// the odd rows and the second in the even rows.
+__device__ __forceinline__ float lane_xor16(float x, bool odd_row) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(odd_row ? r[0] : r[1]);
+}
+
+// Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A,
+// B = 2^LOG2B.
 //   band (tm_buf + zs_off)  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
 //   tm_pre[k][l']           = W_F^{P2 k l'}             (input factor)
 //   tm_post[k1][l'][j]      = W_F^{j (l' + (F/A) k1)}  (between the register passes, P2 > 1)
 // Result U[f][qb0 .. qb0+Qb), stored through an LDS tile in Qb-element runs.
 // The passes are separate (not inlined) functions: inlined into the row loop
 // together they exceed the register file and spill.
-template <int LOG2A, int LOG2F>
+template <int LOG2A, int LOG2F, int LOG2B>
 __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
   constexpr int log2F = LOG2F;
   constexpr int zs_off = team_zs_offset(LOG2A, LOG2F);
@@ -391,7 +416,7 @@ __device__ __noinline__ void team_pass1(int qb0, gfloat2* __restrict__ U) {
 #pragma unroll
   for (int s = 0; s < S / 2; ++s) {
     const int e = 2 * (s * kTeamThreads + threadIdx.x);
-    gstore2(U + (e >> log2Qb) * kTeamB + qb0 + (e & ((1 << log2Qb) - 1)), o[2 * s], o[2 * s + 1]);
+    gstore2(U + ((e >> log2Qb) << LOG2B) + qb0 + (e & ((1 << log2Qb) - 1)), o[2 * s], o[2 * s + 1]);
   }
 }
 
@@ -479,21 +504,86 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
   }
 }
 
-// Calls pass 1 for the row's A = 2^log2A (A <= F/2).
+// Pass 2 at B = 512 for one row: 8 consecutive f starting at f0.  With
+// qb = j + 32 k and m = m1 + 16 m2:
+//   y[f + F m] = c sum_j W_32^{j m2} W_512^{j m1} W_N^{j f}
+//                      sum_k W_16^{k m1} W_N^{32 k f} U[f][qb]
+// Stage 1, thread (g, j) = (tid >> 5, tid & 31): the 16-point DFT over k
+// (lanes of one g read 32 consecutive qb of its scratch row).  Stage 2,
+// thread (g, m1, e) with g = tid & 7, e = (tid >> 4) & 1 and
+// m1 = ((tid >> 3) & 1) + 2 (tid >> 5): the 16-point DFT over j' of the
+// values j = 2 j' + e, E_e[m2'], then the radix-2 step between the lane pair
+// e = 0, 1 (lanes 16 apart):
+//   y[f + F (m1 + 16 m2' + 256 h)] = E_0[m2'] + (-1)^h W_32^{m2'} E_1[m2'],
+// lane e writing h = e.  Lanes g = 0..7 of one (m1, e) write 64-byte runs of
+// the output row straight from the registers.
+//   tm_t2[g][k] = W_N^{32 k (f0 + g)}   tm_wb[m1][j] = W_512^{j m1}
+//   base = c W_N^{j (f0 + g)} for this thread's stage-1 (g, j)
 template <int LOG2F>
+__device__ __forceinline__ void team_pass2_b512(gfloat2* __restrict__ out, int n, int f0, const gfloat2* U,
+                                                float2 base, unsigned* done) {
+  constexpr int JS = 33, XS = 16 * JS + 4;  // exchange [g][m1][j]
+  float2* const xb = tm_buf;
+  const float2* const t2 = tm_t2;
+  const float2* const wb = tm_wb;
+  const int tid = threadIdx.x;
+  {
+    const int g = tid >> 5, j = tid & 31;
+    float2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) v[k] = ld_agent2(U + (size_t)(f0 + g) * 512 + j + 32 * k);
+#pragma unroll
+    for (int k = 1; k < 16; ++k) v[k] = cmul(v[k], t2[g * 16 + k]);
+    dft_reg<16, 1>(v);
+    v[0] = cmul(v[0], base);
+#pragma unroll
+    for (int m1 = 1; m1 < 16; ++m1) v[m1] = cmul(v[m1], cmul(base, wb[m1 * 32 + j]));
+#pragma unroll
+    for (int m1 = 0; m1 < 16; ++m1) xb[g * XS + m1 * JS + j] = v[m1];
+  }
+  __syncthreads();
+  // every thread's reads of U have landed (they fed the exchange above), so
+  // the buffer may be reused; the output stores need not complete
+  if (tid == 0) add_agent(done, 1u);
+  const int g = tid & 7, e = (tid >> 4) & 1, m1 = ((tid >> 3) & 1) + 2 * (tid >> 5);
+  float2 v[16];
+#pragma unroll
+  for (int jp = 0; jp < 16; ++jp) v[jp] = xb[g * XS + m1 * JS + 2 * jp + e];
+  dft_reg<16, 1>(v);  // E_e[m2']
+  const bool odd = e != 0;
+  float2 o[16];
+#pragma unroll
+  for (int m2 = 0; m2 < 16; ++m2) {
+    float2 x = v[m2];
+    if (m2 > 0) {
+      const float2 wx = cmul(x, wb[m2 * 32 + 16]);  // W_32^{m2'} = W_512^{16 m2'}
+      x = odd ? wx : x;
+    }
+    const float2 pr = make_float2(lane_xor16(x.x, odd), lane_xor16(x.y, odd));
+    o[m2] = odd ? csub(pr, x) : cadd(x, pr);
+  }
+  // stores clipped at t >= n by the row's buffer range
+  const __amdgpu_buffer_rsrc_t rows = row_rsrc((float2*)out, n);
+#pragma unroll
+  for (int m2 = 0; m2 < 16; ++m2)
+    bstore_nt(rows, (unsigned)(f0 + g) + ((unsigned)(m1 + 16 * m2 + 256 * e) << LOG2F), o[m2]);
+}
+
+// Calls pass 1 for the row's A = 2^log2A (A <= F/2).
+template <int LOG2F, int LOG2B>
 __device__ __forceinline__ void team_pass1_for(int log2A, int qb0, gfloat2* U) {
   switch (log2A) {
-    case 1: team_pass1<1, LOG2F>(qb0, U); break;
-    case 2: team_pass1<2, LOG2F>(qb0, U); break;
-    case 3: team_pass1<3, LOG2F>(qb0, U); break;
-    case 4: team_pass1<4, LOG2F>(qb0, U); break;
-    case 5: team_pass1<5, LOG2F>(qb0, U); break;
-    case 6: team_pass1<6, LOG2F>(qb0, U); break;
+    case 1: team_pass1<1, LOG2F, LOG2B>(qb0, U); break;
+    case 2: team_pass1<2, LOG2F, LOG2B>(qb0, U); break;
+    case 3: team_pass1<3, LOG2F, LOG2B>(qb0, U); break;
+    case 4: team_pass1<4, LOG2F, LOG2B>(qb0, U); break;
+    case 5: team_pass1<5, LOG2F, LOG2B>(qb0, U); break;
+    case 6: team_pass1<6, LOG2F, LOG2B>(qb0, U); break;
     case 7:
-      if constexpr (LOG2F > 7) team_pass1<7, LOG2F>(qb0, U);
+      if constexpr (LOG2F > 7) team_pass1<7, LOG2F, LOG2B>(qb0, U);
       break;
     default:
-      if constexpr (LOG2F > 8) team_pass1<8, LOG2F>(qb0, U);
+      if constexpr (LOG2F > 8) team_pass1<8, LOG2F, LOG2B>(qb0, U);
       break;
   }
 }
@@ -529,7 +619,6 @@ __device__ __noinline__ void team_direct(const float2* spec, const float* mother
 // their pass 1, and the teams finish at different times); these blocks,
 // which depend on nothing, fill that time.
 __device__ __forceinline__ void team_fill(const TeamArgs& ta, unsigned* next, int* s_next) {
-  const int e0 = ta.fill_end[0], e1 = ta.fill_end[1], e2 = ta.fill_end[2], e3 = ta.fill_end[3];
   for (;;) {
     __syncthreads();  // every thread has read s_next
     if (threadIdx.x == 0) *s_next = (int)atomicAdd(next, 1u);
@@ -538,11 +627,19 @@ __device__ __forceinline__ void team_fill(const TeamArgs& ta, unsigned* next, in
     if (d >= ta.nfill) break;
     const int* const w = (const int*)(ta.fill_items + d);
     const Item it = {ld_const(w), ld_const(w + 1)};
-    const int lm = kFillMinLog2M + (d >= e0) + (d >= e1) + (d >= e2) + (d >= e3);
+    int lm = kFillMinLog2M;
+#pragma unroll
+    for (int g = 0; g < kFillGroups - 1; ++g) lm += d >= ta.fill_end[g];
     const KArgs& k = ta.k;
 #define FCWT_TEAM_DIRECT(L) \
   team_direct<L>(k.spec, k.mother, k.segs, k.rows, k.out, k.outs, k.tw.t1, k.tw.t2, k.tw.s1bits, k.log2N, k.norm, it.row, it.block)
     switch (lm) {
+      case 0: FCWT_TEAM_DIRECT(0); break;
+      case 1: FCWT_TEAM_DIRECT(1); break;
+      case 2: FCWT_TEAM_DIRECT(2); break;
+      case 3: FCWT_TEAM_DIRECT(3); break;
+      case 4: FCWT_TEAM_DIRECT(4); break;
+      case 5: FCWT_TEAM_DIRECT(5); break;
       case 6: FCWT_TEAM_DIRECT(6); break;
       case 7: FCWT_TEAM_DIRECT(7); break;
       case 8: FCWT_TEAM_DIRECT(8); break;
@@ -553,17 +650,23 @@ __device__ __forceinline__ void team_fill(const TeamArgs& ta, unsigned* next, in
   }
 }
 
-// F = 2^LOG2F = N / 256 is a template parameter, so every stride and table
-// index in the pass functions is a compile-time constant.
-template <int LOG2F>
-__global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamArgs ta) {
+// Items per pass of one row of N = 2^log2N points: N / 4096.
+__host__ __device__ constexpr int team_log2i(int log2N) { return log2N - 12; }
+
+// This work-group's walk of its team's sequence in one plan's launch, until
+// the sequence is claimed to its end or the launch aborts (a failed wait).
+// F = 2^LOG2F = N / B and B = 2^LOG2B are template parameters, so every
+// stride and table index in the pass functions is a compile-time constant.
+// s_flag and s_next are the work-group's LDS words for broadcasts.
+template <int LOG2F, int LOG2B>
+__device__ __forceinline__ void team_walk(const TeamArgs& ta, unsigned* const abort, const int team, int& s_flag,
+                                          int& s_next) {
+  static_assert(LOG2B == kTeamLog2B || (LOG2B == 9 && LOG2F == 9), "team split out of range");
   float2* const buf = tm_buf;
   float2* const pre = tm_pre;
   float2* const post = tm_post;
   float2* const t2s = tm_t2;
-  float2* const wb = tm_wb;
-  __shared__ int s_flag, s_next;
-  // plain copies of the arguments (never take the address of the kernel argument)
+  // plain copies of the arguments
   const float2* __restrict__ spec = ta.k.spec;
   const float* __restrict__ mother = ta.k.mother;
   const SegD* __restrict__ segs = ta.k.segs;
@@ -573,17 +676,21 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   constexpr int log2F = LOG2F;
   unsigned* const sync = ta.sync;
   unsigned* const error = ta.error;
-
-  unsigned* const abort = sync + kSyncAbort;
   const long spin = ta.spin;
-  // the team: the XCD this work-group runs on
-  const int team = __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7;
   constexpr int F = 1 << log2F;
   constexpr int log2Qb = 12 - log2F;
-  constexpr int log2I = log2F - 4;  // items per pass per row: F/16
-  for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) {
-    const int k1 = e >> 4, j = e & 15;
-    wb[e] = tw((unsigned)(k1 * j) << log2F);
+  constexpr int log2I = team_log2i(log2F + LOG2B);  // items per pass per row
+  constexpr int log2Fi = 12 - LOG2B;         // f per pass-2 item: 4096/B
+  if constexpr (LOG2B == 9) {
+    for (int e = threadIdx.x; e < 512; e += kTeamThreads) {  // [m1][j] = W_512^{j m1}
+      const int m1 = e >> 5, j = e & 31;
+      tm_wb[e] = tw((unsigned)(m1 * j) << log2F);
+    }
+  } else {
+    for (int e = threadIdx.x; e < kTeamB; e += kTeamThreads) {  // [k1][j] = W_256^{j k1}
+      const int k1 = e >> 4, j = e & 15;
+      tm_wb[e] = tw((unsigned)(k1 * j) << log2F);
+    }
   }
   const int nbuf = ta.nbuf;
   float2* const scratch = ta.scratch + (size_t)team * nbuf * N;
@@ -642,12 +749,12 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
         for (int e = threadIdx.x; e < (R1 << log2Ph); e += kTeamThreads) {
           // [k][l'] = W_F^{P2 k l'} = W_N^{B P2 k l'}; P2 k l' < F
           const int lp = e & ((1 << log2Ph) - 1), k = e >> log2Ph;
-          pre[e] = tw((unsigned)(k * lp) << (log2P2 + kTeamLog2B));
+          pre[e] = tw((unsigned)(k * lp) << (log2P2 + LOG2B));
         }
         if (P2 > 1)
           for (int e = threadIdx.x; e < F; e += kTeamThreads) {  // [k1][l'][j] = W_F^{j (l' + (F/A) k1)}
             const int j = e & (P2 - 1), lp = (e >> log2P2) & ((1 << log2Ph) - 1), k1 = e >> (log2P2 + log2Ph);
-            post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << kTeamLog2B);
+            post[e] = tw((unsigned)((j * (lp + (k1 << log2Ph))) & (F - 1)) << LOG2B);
           }
       }
       // the row's windowed band on this item's bins, into LDS: at most
@@ -664,7 +771,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       for (int u = 0; u < kPer; ++u) {
         const int e = u * kTeamThreads + threadIdx.x;
         const int ql = e & ((1 << log2Qb) - 1), qf = e >> log2Qb;  // consecutive lanes, consecutive q
-        qs[u] = qb0 + ql + (qf << kTeamLog2B);  // qf < A <= F/2: q < N/2
+        qs[u] = qb0 + ql + (qf << LOG2B);  // qf < A <= F/2: q < N/2
         idx[u] = e < total ? window_index_d(qs[u], nseg, s0, s1, segs + seg_begin, maxidx) : -1;
       }
       float2 xs[kPer];
@@ -687,7 +794,7 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       } else {
         __syncthreads();
       }
-      team_pass1_for<LOG2F>(log2A, qb0, U);
+      team_pass1_for<LOG2F, LOG2B>(log2A, qb0, U);
       // every thread's pass-1 stores are complete before the arrival is counted
       wait_stores();
       __syncthreads();
@@ -698,16 +805,25 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
       // pass-2 item's end releases its buffer and finishes rows (0.5-1% of
       // the step, measured)
       __builtin_amdgcn_s_setprio(2);
-      const int f0 = blk << 4;
-      {
+      const int f0 = blk << log2Fi;
+      float2 base;  // W_N^{j f} times the row normalisation, for this thread's (g, j)
+      if constexpr (LOG2B == 9) {
+        if (threadIdx.x < 128) {
+          const int g = threadIdx.x >> 4, k = threadIdx.x & 15;  // t2s[g][k] = W_N^{32 k (f0 + g)}
+          t2s[threadIdx.x] = tw((unsigned)((32 * k * (f0 + g)) & (N - 1)));
+        }
+        base = cscale(tw((unsigned)((threadIdx.x & 31) * (f0 + (threadIdx.x >> 5)))), norm);
+      } else {
         const int g = threadIdx.x >> 4, k = threadIdx.x & 15;  // t2s[g][k] = W_N^{16 k (f0 + g)}
         t2s[threadIdx.x] = tw((unsigned)((16 * k * (f0 + g)) & (N - 1)));
+        base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       }
-      // W_N^{j f} times the row normalisation
-      const float2 base = cscale(tw((unsigned)((threadIdx.x & 15) * (f0 + (threadIdx.x >> 4)))), norm);
       // every pass-1 item of the row has stored its part (the wait ends in a barrier)
       if (!team_wait(dep, (unsigned)((gen + 1) << log2I), first, &s_flag, abort, error, spin)) break;
-      team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
+      if constexpr (LOG2B == 9)
+        team_pass2_b512<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
+      else
+        team_pass2<LOG2F>((gfloat2*)out, n, f0, U, base, arrive + 64 * b + 32);
       __builtin_amdgcn_s_setprio(0);
     }
     // publishes the next item, and every thread is done with this item's LDS
@@ -715,8 +831,84 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
     __syncthreads();
     it = __builtin_amdgcn_readfirstlane(s_next);
   }
-  if (ta.nfill > 0) team_fill(ta, sync + kSyncFill, &s_next);
-  team_exit(sync, error, ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, log2I, ta.k.out, ta.k.outs, &s_flag);
+}
+
+// The team: the XCD this work-group runs on (its XCC id).
+__device__ __forceinline__ int team_of_workgroup() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7; }
+
+// One plan's wide rows (and, once a member's sequence is claimed to its
+// end, its direct-row blocks).  The launch's abort and exit words are the
+// plan's own sync words.
+template <int LOG2F, int LOG2B>
+__global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamArgs ta) {
+  __shared__ int s_flag, s_next;
+  unsigned* const abort = ta.sync + kSyncAbort;
+  team_walk<LOG2F, LOG2B>(ta, abort, team_of_workgroup(), s_flag, s_next);
+  if (ta.nfill > 0) team_fill(ta, ta.sync + kSyncFill, &s_next);
+  if (team_exit_last(ta.sync, &s_flag)) {
+    if (threadIdx.x == 0) {
+      team_check_claims(ta.sync, abort, ta.error, ta.team_blocks, ta.nteams, team_log2i(LOG2F + LOG2B));
+      s_flag = ld_agent(abort) != 0;
+    }
+    __syncthreads();
+    if (s_flag) team_nan_rows(ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, ta.k.out, ta.k.outs);
+  }
+}
+
+// One launch for the wide rows of several plans of a batch, one plan per
+// padded size: seg[0..3] hold the plans of N = 2^18, 2^17, 2^16, 2^15 (bit s
+// of `used` set when seg[s] holds one).  Every work-group walks its team's
+// sequence of each plan in turn, largest N first: once a plan's sequence is
+// claimed to its end the member goes on to the next plan's, so the plans'
+// sequences overlap at their ends instead of each launch draining alone.
+// Each plan keeps its own scratch, item counters and buffer counters, so a
+// plan's items depend only on earlier items of the same plan and every
+// plan's walk progresses as in its own launch.  After the last plan, the
+// members take the direct-row blocks of every plan.  The abort and exit words
+// are the launch's own (ctl): a failed wait in any plan ends the whole
+// launch, every plan reports it, and every plan's team rows are filled with
+// NaN.
+constexpr int kTeamMaxSegs = 4;
+struct TeamMulti {
+  TeamArgs seg[kTeamMaxSegs];
+  unsigned* ctl;  // [kSyncAbort] abort code, [kSyncExited] finished work-groups; zeroed before launch
+  int used;
+};
+
+__global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team_multi(TeamMulti m) {
+  __shared__ int s_flag, s_next;
+  unsigned* const abort = m.ctl + kSyncAbort;
+  const int team = team_of_workgroup();
+  const int used = m.used;
+  if (used & 1) team_walk<9, 9>(m.seg[0], abort, team, s_flag, s_next);
+  if (used & 2) team_walk<9, 8>(m.seg[1], abort, team, s_flag, s_next);
+  if (used & 4) team_walk<8, 8>(m.seg[2], abort, team, s_flag, s_next);
+  if (used & 8) team_walk<7, 8>(m.seg[3], abort, team, s_flag, s_next);
+#pragma unroll
+  for (int s = 0; s < kTeamMaxSegs; ++s)
+    if ((used >> s & 1) && m.seg[s].nfill > 0) team_fill(m.seg[s], m.seg[s].sync + kSyncFill, &s_next);
+  if (team_exit_last(m.ctl, &s_flag)) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int s = 0; s < kTeamMaxSegs; ++s)
+        if (used >> s & 1)
+          team_check_claims(m.seg[s].sync, abort, m.seg[s].error, m.seg[s].team_blocks, m.seg[s].nteams,
+                            team_log2i(18 - s));
+      const unsigned code = ld_agent(abort);
+      if (code != 0)
+#pragma unroll
+        for (int s = 0; s < kTeamMaxSegs; ++s)
+          if (used >> s & 1) atomicOr(m.seg[s].error, code);
+      s_flag = code != 0;
+    }
+    __syncthreads();
+    if (s_flag)
+#pragma unroll
+      for (int s = 0; s < kTeamMaxSegs; ++s)
+        if (used >> s & 1)
+          team_nan_rows(m.seg[s].team_desc, m.seg[s].team_blocks, m.seg[s].seq_stride, m.seg[s].nteams,
+                        m.seg[s].k.out, m.seg[s].k.outs);
+  }
 }
 
 }  // namespace fcwt_hip

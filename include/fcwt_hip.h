@@ -262,6 +262,14 @@ int fcwt_hip_batch_check(fcwt_hip_batch batch);
 /* Plans (distinct padded sizes) in the batch and rows on the XCD-team kernel. */
 int fcwt_hip_batch_shape(fcwt_hip_batch batch, int* groups, int* team_rows);
 
+/* Launches of row kernels (direct, XCD-team, column and two-pass kernels;
+   not the forward transforms or the fills that clear counters) that one
+   execute of the batch makes.  When two or more padded sizes have rows on
+   the XCD-team kernel, one launch takes the team rows and the direct rows of
+   all of them.  Introspection of this build's schedule; no counterpart in the
+   reference, which makes one FCWT::cwt call per signal (fcwt.cpp:447-529). */
+int fcwt_hip_batch_row_launches(fcwt_hip_batch batch, int* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,3 +123,64 @@ def test_pass2_512_point_decomposition():
             o = pr - x[tid] if e else x[tid] + pr
             got[f0 + g + F * (m1 + 16 * np.arange(16) + 256 * e)] = o
     assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def test_team_pass2_512_point_decomposition():
+    """team_pass2_b512 (rows_team.h): 256 threads per 8 f of a team's scratch
+    U[f][qb] (row stride 512), qb = j + 32 k, m = m1 + 16 m2.  Stage 1 (thread
+    g = tid >> 5, j = tid & 31): W_N^{32 k f} from the [g][k] table, a 16-point
+    DFT over k, then c W_N^{j f} W_512^{j m1}; the exchange xb[g][m1][j] at
+    g * 532 + m1 * 33 + j; stage 2 (g = tid & 7, e = (tid >> 4) & 1,
+    m1 = ((tid >> 3) & 1) + 2 (tid >> 5)): a 16-point DFT over j' of
+    xb[g][m1][2 j' + e]; lane e = 1 multiplies by W_32^{m2'}, the lanes 16
+    apart (tid ^ 16: same g and m1, the other e) swap, e = 0 writes
+    E_0 + W E_1 at m = m1 + 16 m2' and e = 1 writes E_0 - W E_1 at m + 256.
+    Every exchange index stays inside the 4768-value LDS buffer.  Against
+    y[f + F m] = sum_qb W_512^{qb m} W_N^{qb f} U[f][qb]."""
+    rng = np.random.default_rng(7)
+    log2F = 5
+    F, B = 1 << log2F, 512
+    N = B * F
+    U = rng.standard_normal((F, B)) + 1j * rng.standard_normal((F, B))
+
+    def W(X, r):
+        return np.exp(2j * np.pi * np.asarray(r) / X)
+
+    ref = np.zeros(N, complex)
+    for f in range(F):
+        ref[f + F * np.arange(B)] = np.fft.ifft(U[f] * W(N, np.arange(B) * f)) * B
+    wb = W(N, (np.arange(16)[:, None] * np.arange(32)[None, :]) << log2F)  # [m1][j]
+    JS, XS = 33, 16 * 33 + 4
+    got = np.full(N, np.nan, complex)
+    k = np.arange(16)
+    tids = np.arange(256)
+    assert (7 * XS + 15 * JS + 31) < 4768
+    for f0 in range(0, F, 8):
+        xb = np.full(4768, np.nan, complex)
+        for tid in tids:
+            g, j = tid >> 5, tid & 31
+            f = f0 + g
+            v = U[f, j + 32 * k] * W(N, (32 * k * f) % N)
+            v = np.fft.ifft(v) * 16
+            v = v * W(N, j * f) * wb[:, j]
+            xb[g * XS + k * JS + j] = v  # k here runs over m1
+        E = np.zeros((256, 16), complex)
+        for tid in tids:
+            g, e, m1 = tid & 7, (tid >> 4) & 1, ((tid >> 3) & 1) + 2 * (tid >> 5)
+            E[tid] = np.fft.ifft(xb[g * XS + m1 * JS + 2 * np.arange(16) + e]) * 16
+        x = E.copy()
+        odd = ((tids >> 4) & 1) == 1
+        x[odd] *= wb[np.arange(16), 16][None, :]
+        seen = set()
+        for tid in tids:
+            g, e, m1 = tid & 7, (tid >> 4) & 1, ((tid >> 3) & 1) + 2 * (tid >> 5)
+            p = tid ^ 16
+            assert (p & 7, ((p >> 3) & 1) + 2 * (p >> 5)) == (g, m1) and ((p >> 4) & 1) != e
+            pr = x[p]
+            o = pr - x[tid] if e else x[tid] + pr
+            t = f0 + g + F * (m1 + 16 * np.arange(16) + 256 * e)
+            seen.update(t.tolist())
+            got[t] = o
+        assert len(seen) == 8 * 512
+    assert not np.isnan(got).any()
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12

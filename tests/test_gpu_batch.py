@@ -113,3 +113,67 @@ def test_graph_replays_match_eager_execution(monkeypatch):
         for g, r in zip(run(out_a), ref):
             assert np.array_equal(g, r), k
     assert b.graph_launches() == 6
+
+
+def _sizes_batch(seed=9):
+    """Channels of every padded size the XCD-team kernel takes (2^15 .. 2^18),
+    sigma 6 LOGSCALES 10 Hz .. 10 kHz at fs 30 kHz: wide rows on the team
+    kernel and direct rows of every band from 2^1 up."""
+    lengths = [17_000, 30_011, 33_000, 60_000, 65_536, 90_001, 131_072, 140_000, 200_000]
+    xs = [tones_noise(n, fs=30_000, seed=n, fmax=12_000) for n in lengths]
+    sc = O.scales(O.LOGSCALES, 30_000, 10, 10_000, 40)
+    return lengths, xs, sc
+
+
+def test_one_team_launch_for_every_padded_size(monkeypatch):
+    """A batch whose signals pad to 2^15, 2^16, 2^17 and 2^18 makes one
+    row-kernel launch per execute (k_rows_team_multi: the team rows of the four
+    plans, then their direct rows); each signal's output equals its
+    one-signal plan's and the per-size team launches'
+    (FCWT_HIP_BATCH_TEAM_LAUNCHES=separate) bit for bit; sampled rows of every
+    size against the oracle."""
+    lengths, xs, sc = _sizes_batch()
+    b = fcwt_amd.Batch(lengths, sc, 6.0)
+    shp = b.shape()
+    assert shp["plans"] == 4 and shp["team_rows"] > 0 and shp["row_launches"] == 1, shp
+    got = _run_batch(b, xs, [sc.size] * len(xs))
+    again = _run_batch(b, xs, [sc.size] * len(xs))  # counters cleared before every launch
+    monkeypatch.setenv("FCWT_HIP_BATCH_TEAM_LAUNCHES", "separate")
+    sep = fcwt_amd.Batch(lengths, sc, 6.0)
+    assert sep.shape()["row_launches"] > 1
+    ref_sep = _run_batch(sep, xs, [sc.size] * len(xs))
+    for k, x in enumerate(xs):
+        assert np.array_equal(got[k], again[k]), x.size
+        assert np.array_equal(got[k], ref_sep[k]), x.size
+        assert np.array_equal(got[k], run_device(x, sc, 6.0)), x.size
+    N = [1 << O.find2power(n) for n in lengths]
+    rows = [0, 5, 12, 20, 28, 39]
+    for k in (0, 3, 5, 8):
+        ref = O.cwt_fp64(xs[k], sc, 6.0, rows=rows)
+        err = O.rel_l2_rows(got[k][rows], ref)
+        assert err.max() <= TOL, (lengths[k], N[k], rows[int(np.argmax(err))], err.max())
+
+
+def test_one_team_launch_failure_is_reported_by_every_plan(monkeypatch):
+    """A spin limit of zero polls fails the launch's first team wait: every
+    plan of the batch reports the failure, every team row of the launch is
+    NaN, and the next execute (normal spin) is correct."""
+    from fcwt_amd import _native as nat
+
+    lengths, xs, sc = _sizes_batch(seed=10)
+    monkeypatch.setenv("FCWT_HIP_TEAM_SPIN", "0")
+    b = fcwt_amd.Batch(lengths, sc, 6.0)
+    assert b.shape()["row_launches"] == 1
+    st = torch.cuda.current_stream()
+    dx = [torch.from_numpy(x).cuda() for x in xs]
+    outs = [torch.zeros((sc.size, x.size), dtype=torch.complex64, device="cuda") for x in xs]
+    b.execute_device([d.data_ptr() for d in dx], [o.data_ptr() for o in outs], st.cuda_stream)
+    torch.cuda.synchronize()
+    with pytest.raises(nat.FcwtError):
+        b.check()
+    b.check()  # reported once
+    assert all(torch.isnan(o).any().item() for o in outs)  # every size's team rows
+    monkeypatch.delenv("FCWT_HIP_TEAM_SPIN")
+    got = _run_batch(b, xs, [sc.size] * len(xs))
+    for k in (0, 4, 8):
+        assert np.array_equal(got[k], run_device(xs[k], sc, 6.0)), lengths[k]

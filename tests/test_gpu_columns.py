@@ -35,9 +35,13 @@ def _rows_spanning_bands(n, sc, fb=2.0, k=10):
 
 
 @pytest.mark.parametrize("n,fs,fn", [(150_000, 1000, 40), (600_000, 1000, 32), (3_000_000, 1000, 24)])
-def test_column_path_against_oracle(n, fs, fn):
+def test_column_path_against_oracle(n, fs, fn, monkeypatch):
     """N = 2^18, 2^20, 2^22: LINFREQS 1..fs/2 with sigma 4 (the long-signal
-    workload's scale family), so bands run from a few thousand bins to N/2."""
+    workload's scale family), so bands run from a few thousand bins to N/2
+    (at N = 2^18 with the team path, which takes those rows by default,
+    switched off)."""
+    if n <= (1 << 18):
+        monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
     x = tones_noise(n, fs=fs, fmax=0.45 * fs, seed=n)
     sc = O.scales(O.LINFREQS, fs, 1, fs / 2, fn)
     p = fcwt_amd.Plan(n, sc, 4.0)
@@ -101,7 +105,9 @@ def test_column_path_low_frequency_rows(monkeypatch):
 @pytest.mark.parametrize("n", [10_000, 150_000, 1_500_000])
 def test_column_path_matches_two_pass_kernels(n, monkeypatch):
     """The same plan through the generic two-pass kernels: same rows, the
-    float32 results agree to float32 rounding."""
+    float32 results agree to float32 rounding (the team path, which takes
+    the wide rows at N = 2^18 by default, switched off)."""
+    monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
     x = tones_noise(n, seed=7)
     sc = O.scales(O.LINFREQS, 1000, 1, 500, 16)
     cols = run_device(x, sc, 4.0)

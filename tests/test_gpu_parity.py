@@ -103,11 +103,12 @@ def test_headline_100k_x_3000_rows():
 
 
 @pytest.mark.parametrize("teams", [True, False])
-@pytest.mark.parametrize("n", [20_000, 50_000, 50_001, 100_000])
+@pytest.mark.parametrize("n", [20_000, 50_000, 50_001, 100_000, 150_000, 200_001])
 def test_wide_band_paths(n, teams, monkeypatch):
-    """N = 2^15, 2^16, 2^17 with bands up to N/2, through the XCD-team kernel
-    and (with the team path switched off) the two-pass fallback; an odd n
-    takes the team kernel's single-value output stores."""
+    """N = 2^15 .. 2^18 with bands up to N/2, through the XCD-team kernel
+    (at N = 2^18 its 512 x 512 split with the 512-point pass 2) and (with the
+    team path switched off) the two-pass fallback, or at N = 2^18 the column
+    path; an odd n takes the team kernel's single-value output stores."""
     if not teams:
         monkeypatch.setenv("FCWT_HIP_NO_TEAMS", "1")
     x = tones_noise(n, seed=n)
@@ -122,13 +123,15 @@ def test_wide_band_paths(n, teams, monkeypatch):
     assert err.max() <= TOL, f"row {int(np.argmax(err))}: rel-L2 {err.max():.3e}"
 
 
+@pytest.mark.parametrize("n", [100_000, 180_000])
 @pytest.mark.parametrize("nbuf", [2, 3, 12])
-def test_team_buffer_counts(nbuf, monkeypatch):
+def test_team_buffer_counts(nbuf, n, monkeypatch):
     """The XCD-team kernel with the fewest scratch buffers (2 and 3: pass 2
     one step behind pass 1, no slack) and the most (12), through the
-    environment override: same rows as the default, within the bar."""
+    environment override, at N = 2^17 and 2^18: same rows as the default,
+    within the bar."""
     monkeypatch.setenv("FCWT_HIP_TEAM_BUFFERS", str(nbuf))
-    x = tones_noise(100_000, seed=40 + nbuf)
+    x = tones_noise(n, seed=40 + nbuf)
     sc = O.scales(O.LOGSCALES, 1000, 1, 450, 120)
     p = fcwt_amd.Plan(x.size, sc, 2.0)
     assert p.row_split()["team"] > 0
