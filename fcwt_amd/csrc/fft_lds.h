@@ -52,6 +52,13 @@ __device__ __forceinline__ void bstore_nt(__amdgpu_buffer_rsrc_t r, unsigned idx
   const u32x2 u = {__float_as_uint(v.x), __float_as_uint(v.y)};
   __builtin_amdgcn_raw_buffer_store_b64(u, r, idx * 8u, 0, 2);
 }
+// row[idx] = v without a cache hint: the L2 keeps the line, so that the
+// other half of a 128-byte line, written by another work-group shortly
+// after, joins it before the line goes to memory
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned idx, float2 v) {
+  const u32x2 u = {__float_as_uint(v.x), __float_as_uint(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(u, r, idx * 8u, 0, 0);
+}
 // row[idx] = a, row[idx + 1] = b in one 16-byte store (idx even, the row
 // 16-byte aligned; both in range or both out: the caller keeps pairs from
 // straddling the end)

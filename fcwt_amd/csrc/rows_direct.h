@@ -190,7 +190,13 @@ __device__ __forceinline__ void direct_block(const KArgs& a, const Item it, floa
 #pragma unroll
         for (int s = 0; s < S; ++s) {
           const int e = s * NT + tid;
-          bstore_nt(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
+          // runs shorter than 128 bytes (M = 1024 in the team launch) go
+          // without the non-temporal hint, so that the L2 joins the runs of
+          // neighbouring blocks (1.3% of the 384-channel batch, measured)
+          if constexpr (TEAM && C::G < 16)
+            bstore(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
+          else
+            bstore_nt(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
         }
       } else {
         for (int e = tid; e < MH * G; e += NT) {
