@@ -424,7 +424,9 @@ TRACES = [
     ("trace_batched", "cwt_batched_384_channels", "384 channels x 200 LOGSCALES (sigma 6)",
      "--workload batched --steps 3 --warmup 2",
      "Calls include the plain and batched runs of bench.py (warm-up and timed; the batch warms up with two "
-     "executes, the second of which captures its HIP graph)."),
+     "executes, the second of which captures its HIP graph). Each batched execute makes one row-kernel launch, "
+     "`k_rows_team_multi` (the team rows of the four padded sizes and then every direct row); the other row "
+     "kernels below are the 384 plain calls'."),
     ("trace_adjoint", "adjoint_64_x_16384_x_128", "adjoint, 64 signals x 16,384 x 128 LOGSCALES (sigma 2)",
      "--workload adjoint --steps 3 --warmup 1",
      "Calls include the plain forward, the batched forward and the adjoint runs of bench.py (warm-up and timed)."),
