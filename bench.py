@@ -30,6 +30,11 @@ scales of the same workload (all 3000 by default).  The reference's own
 FFTW+OpenMP build cannot be made here (FFTW exists only as prebuilt binaries
 inside the reference).
 
+The metric's other half, the batched and adjoint forms against the plain
+per-signal GPU call, is measured after the headline on the batched and
+adjoint workloads below (5 timed steps after 3 each) and reported in the same
+line under config.secondary (`--no-secondary` skips it).
+
 Other workloads (each prints its own JSON line; not the headline metric):
   --workload long      n = 2^22 AR(2) + 3-segment tone, Morlet sigma 4, 1024
                        LINFREQS over [0.001, 0.5] fs (34.4 GB of output)
@@ -42,7 +47,7 @@ Other workloads (each prints its own JSON line; not the headline metric):
                        numpy array), pipelined and as one plain copy
 
 Usage:
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-rows R] [--no-cpu] [--no-tune] [--no-secondary]
   python bench.py --copy-probe MIB     one device copy of MIB MiB (counter calibration)
   python bench.py --summarize-profiles DIR
         write profiles/ summaries from the rocprofv3 CSV output under DIR:
@@ -415,7 +420,7 @@ TRACE_CMD = ("cd /tmp && export TMPDIR=/tmp && timeout -k 10 300 rocprofv3 --ker
 
 # kernel traces summarized under profiles/: trace directory, file tag, title, bench.py arguments, note
 TRACES = [
-    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 100 --warmup 50 --no-cpu --no-tune",
+    ("trace", f"cwt_{WORKLOAD_TAG}", "cwt 100k x 3000", "--steps 100 --warmup 50 --no-cpu --no-tune --no-secondary",
      "Calls include the 50 warm-up steps, the 100 timed steps and the 10 steps of the per-kernel event pass of "
      "bench.py."),
     ("trace_long", "cwt_long_4194304_x_1024", "long signal, n = 2^22 x 1024 LINFREQS (sigma 4)",
@@ -547,7 +552,7 @@ def summarize_sq(root):
     out = {"workload": WORKLOAD_TAG, "N": N, "n": n, "kernels": {}}
     cmd = ("cd /tmp && export TMPDIR=/tmp && timeout -s KILL 90 rocprofv3 --pmc {c} --output-format csv "
            "-d $GRAFT_REPO_ROOT/gpurun_out/prof/{d} -o run -- python3 $GRAFT_REPO_ROOT/bench.py --steps 2 "
-           "--warmup 1 --no-cpu --no-tune")
+           "--warmup 1 --no-cpu --no-tune --no-secondary")
     out["commands"] = [cmd.format(c=" ".join(v), d=k) for k, v in SQ_PASSES.items()]
     for kname, kind in SQ_KERNELS:
         if kname not in per:
@@ -644,8 +649,8 @@ def summarize_pmc(root):
         "fetch_scale": f_fetch, "write_scale": f_write,
         "fetch_bytes_per_launch": fetch_b, "write_bytes_per_launch": write_b,
         "traffic_bytes_per_launch": fetch_b + write_b,
-        "commands": [cmd.format(c="FETCH_SIZE", d="pmc_fetch", a="--steps 2 --warmup 1 --no-cpu --no-tune"),
-                     cmd.format(c="WRITE_SIZE", d="pmc_write", a="--steps 2 --warmup 1 --no-cpu --no-tune"),
+        "commands": [cmd.format(c="FETCH_SIZE", d="pmc_fetch", a="--steps 2 --warmup 1 --no-cpu --no-tune --no-secondary"),
+                     cmd.format(c="WRITE_SIZE", d="pmc_write", a="--steps 2 --warmup 1 --no-cpu --no-tune --no-secondary"),
                      cmd.format(c="FETCH_SIZE", d="copy_fetch", a="--copy-probe 1024"),
                      cmd.format(c="WRITE_SIZE", d="copy_write", a="--copy-probe 1024")],
         "summarized_by": "python bench.py --summarize-profiles gpurun_out/prof",
@@ -681,6 +686,8 @@ def main():
     ap.add_argument("--cpu-rows", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the tuned-schedule comparison")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the batched and adjoint workloads the headline line reports beside its metric")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB")
     ap.add_argument("--summarize-profiles", default="", metavar="DIR")
     ap.add_argument("--workload", default="headline", choices=["headline", "long", "batched", "adjoint", "host"])
@@ -827,6 +834,31 @@ def main():
                      "fill_write_gbs": round(fill_gbs, 1),
                      "kernels": kernels},
     }
+    if not args.no_secondary:
+        # the metric's other half: the batched and adjoint forms against the
+        # plain per-signal GPU call, on their own workloads (5 timed steps
+        # after 3 warm-up steps each; `--workload batched|adjoint` prints
+        # each workload's full line)
+        plan.close()
+        del out
+        torch.cuda.empty_cache()
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 5, 3
+        bt = workload_batched(sub)
+        torch.cuda.empty_cache()
+        ad = workload_adjoint(sub)
+        torch.cuda.empty_cache()
+        result["config"]["secondary"] = {
+            "batched": {"workload": "384 channels, n ~ U[20000, 200000], 200 LOGSCALES, sigma 6, 66.7 GB output",
+                        "ms": bt["ms_per_step"], "gbs": bt["value"], "plain_ms": bt["config"]["plain_ms"],
+                        "batched_vs_plain_speedup": bt["config"]["batched_vs_plain_speedup"],
+                        "row_launches_per_execute": bt["config"]["plans"]["row_launches"]},
+            "adjoint": {"workload": "64 x 16384 x 128 LOGSCALES, sigma 2: input gradient from 1.07 GB of output "
+                                    "gradient", "ms": ad["ms_per_step"],
+                        "plain_forward_ms": ad["config"]["plain_forward_ms"],
+                        "batched_forward_ms": ad["config"]["batched_forward_ms"],
+                        "adjoint_vs_plain_forward_speedup": ad["config"]["adjoint_vs_plain_forward_speedup"],
+                        "batched_forward_vs_plain_speedup": ad["config"]["batched_forward_vs_plain_speedup"]}}
     if not args.no_cpu:
         rows = min(args.cpu_rows, fn)
         threads = host_threads()
