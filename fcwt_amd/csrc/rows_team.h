@@ -704,6 +704,10 @@ __device__ __forceinline__ void team_walk(const TeamArgs& ta, unsigned* const ab
   // items are claimed in sequence order from the team's counter; an item
   // claims its successor when it starts, so the claim's latency is hidden
   unsigned* const next_item = sync + 64 * (1 + ta.nteams * kTeamBuffers + team);
+  // every wave has read the broadcast words (s_next, s_flag) of what ran
+  // before this walk (another plan's walk in k_rows_team_multi) before
+  // thread 0 writes them again
+  __syncthreads();
   if (threadIdx.x == 0) s_next = (int)atomicAdd(next_item, 1u);
   __syncthreads();
   int it = __builtin_amdgcn_readfirstlane(s_next);
@@ -848,13 +852,16 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team(TeamAr
   unsigned* const abort = ta.sync + kSyncAbort;
   team_walk<LOG2F, LOG2B>(ta, abort, team_of_workgroup(), s_flag, s_next);
   if (ta.nfill > 0) team_fill(ta, ta.sync + kSyncFill, &s_next);
+  // (s_flag holds the exit's broadcast until every wave has read it; the
+  // abort broadcast goes through s_next, which every wave read before the
+  // exit's barrier)
   if (team_exit_last(ta.sync, &s_flag)) {
     if (threadIdx.x == 0) {
       team_check_claims(ta.sync, abort, ta.error, ta.team_blocks, ta.nteams, team_log2i(LOG2F + LOG2B));
-      s_flag = ld_agent(abort) != 0;
+      s_next = ld_agent(abort) != 0;
     }
     __syncthreads();
-    if (s_flag) team_nan_rows(ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, ta.k.out, ta.k.outs);
+    if (s_next) team_nan_rows(ta.team_desc, ta.team_blocks, ta.seq_stride, ta.nteams, ta.k.out, ta.k.outs);
   }
 }
 
@@ -902,10 +909,10 @@ __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team_multi(
 #pragma unroll
         for (int s = 0; s < kTeamMaxSegs; ++s)
           if (used >> s & 1) atomicOr(m.seg[s].error, code);
-      s_flag = code != 0;
+      s_next = code != 0;  // as in k_rows_team: not s_flag, which a wave may still be reading
     }
     __syncthreads();
-    if (s_flag)
+    if (s_next)
 #pragma unroll
       for (int s = 0; s < kTeamMaxSegs; ++s)
         if (used >> s & 1)
