@@ -433,7 +433,7 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
   float2* const buf = tm_buf;
   const float2* const t2 = tm_t2;
   const float2* const wb = tm_wb;
-  constexpr int R1 = 16, P2 = 16, XS = R1 * (P2 + 1), OS = 17;
+  constexpr int R1 = 16, P2 = 16, XS = 16 * (P2 + 1);
   const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
   float2 v[R1];
 #pragma unroll
@@ -445,63 +445,30 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
 #pragma unroll
   for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wb[k1 * P2 + j]));
 #pragma unroll
-  for (int k1 = 0; k1 < R1; ++k1) buf[g * XS + k1 * (P2 + 1) + j] = v[k1];
-  wave_lds_order();  // the 16 threads of one f are in one wavefront
-  float2 w[P2];
-#pragma unroll
-  for (int jj = 0; jj < P2; ++jj) w[jj] = buf[g * XS + j * (P2 + 1) + jj];
-  dft_reg<P2, 1>(w);
-  // m = j + 16 k2; transpose to [m][g] so that each output run is 16
-  // consecutive f (128 bytes): 32-byte runs written straight from the
-  // registers cost a fifth of the kernel's time
+  for (int k1 = 0; k1 < R1; ++k1) buf[k1 * XS + g * (P2 + 1) + j] = v[k1];  // [k1][g][j]
   __syncthreads();
   // every thread's reads of U have landed (they fed the exchange above), so
   // the buffer may be reused; the output stores need not complete
   if (threadIdx.x == 0) add_agent(done, 1u);
-  // outputs m >= mlim have t = f + F m >= n for every f of the item: they
-  // are neither moved through LDS nor stored
+  // The second stage by thread (g2, k1) = (tid & 15, tid >> 4): the DFT over
+  // j gives m = k1 + 16 k2 at f = f0 + g2, so the 16 lanes of one k1 store a
+  // 128-byte run of the row straight from the registers, with no output
+  // transpose through LDS (the exchange above is the only one; against a
+  // wave-local exchange, an LDS transpose and 16-byte pair stores this is
+  // 1.7% of the headline step, measured).  Stores at t >= n are clipped by
+  // the row's buffer range; the rows m >= mlim of every f of the item are
+  // not stored.
+  const int g2 = threadIdx.x & 15, k1s = threadIdx.x >> 4;
+  float2 w[P2];
+#pragma unroll
+  for (int jj = 0; jj < P2; ++jj) w[jj] = buf[k1s * XS + g2 * (P2 + 1) + jj];
+  dft_reg<P2, 1>(w);
+  const __amdgpu_buffer_rsrc_t rows = row_rsrc((float2*)out, n);
   const int mlim = __builtin_amdgcn_readfirstlane((n - f0 + (1 << LOG2F) - 1) >> LOG2F);
-  const int slim = (mlim + 15) >> 4;  // rows m of the tile in use, in steps of 16
+  const int slim = (mlim + 15) >> 4;
 #pragma unroll
   for (int k2 = 0; k2 < P2; ++k2)
-    if (k2 < slim) buf[(j + R1 * k2) * OS + g] = w[k2];
-  __syncthreads();
-  // every LDS read first, then the stores, which the row's buffer range
-  // clips at t >= n; pairs of consecutive f (16-byte stores) when the row
-  // is 16-byte aligned and even in length, so that no pair straddles t = n
-  constexpr int S = (kTeamB * 16) / kTeamThreads;
-  float2 o[S];
-  const __amdgpu_buffer_rsrc_t rows = row_rsrc((float2*)out, n);
-  if (((n & 1) | ((unsigned)(size_t)out & 15u)) == 0) {
-    // pair e = s * 256 + tid: m = e >> 3 (32 s <= m < 32 s + 32), f = f0 + 2 (e & 7);
-    // rows m in [16 slim, mlim rounded up to 32) hold stale values whose
-    // stores fall at t >= n and are dropped
-    const int plim = (mlim + 31) >> 5;
-#pragma unroll
-    for (int s = 0; s < S / 2; ++s) {
-      const int e = s * kTeamThreads + threadIdx.x;
-      if (s < plim) {
-        o[2 * s] = buf[(e >> 3) * OS + 2 * (e & 7)];
-        o[2 * s + 1] = buf[(e >> 3) * OS + 2 * (e & 7) + 1];
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < S / 2; ++s) {
-      const int e = s * kTeamThreads + threadIdx.x;
-      if (s < plim) bstore2_nt(rows, (unsigned)(f0 + 2 * (e & 7) + ((e >> 3) << LOG2F)), o[2 * s], o[2 * s + 1]);
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int e = s * kTeamThreads + threadIdx.x;
-      if (s < slim) o[s] = buf[(e >> 4) * OS + (e & 15)];
-    }
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const int e = s * kTeamThreads + threadIdx.x;
-      if (s < slim) bstore_nt(rows, (unsigned)(f0 + (e & 15) + ((e >> 4) << LOG2F)), o[s]);
-    }
-  }
+    if (k2 < slim) bstore_nt(rows, (unsigned)(f0 + g2 + ((k1s + 16 * k2) << LOG2F)), w[k2]);
 }
 
 // Pass 2 at B = 512 for one row: 8 consecutive f starting at f0.  With
