@@ -438,24 +438,12 @@ __device__ __forceinline__ void cols_pass2(const BigArgs& a, int y, int fblk, in
 #pragma unroll
   for (int jj = 0; jj < 16; ++jj) v[jj] = xb[k1 * XS + g * 17 + jj];
   dft_reg<16, 1>(v);  // over j: output k2, m = k1 + 16 k2
-  __syncthreads();
-#pragma unroll
-  for (int k2 = 0; k2 < 16; ++k2) xb[(k1 + 16 * k2) * 17 + g] = v[k2];
-  __syncthreads();
-  // every LDS read first, then the stores, clipped at t >= n by the row's
-  // buffer range
-  float2 o[16];
-#pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int e = s * 256 + tid;
-    o[s] = xb[(e >> 4) * 17 + (e & 15)];
-  }
+  // lanes g = 0..15 of one k1 hold 16 consecutive f: each store writes
+  // 128-byte runs of the row straight from the registers (no transpose
+  // through LDS), clipped at t >= n by the row's buffer range
   const __amdgpu_buffer_rsrc_t rr = row_rsrc(row_out(a.k, r), r.n);
 #pragma unroll
-  for (int s = 0; s < 16; ++s) {
-    const int e = s * 256 + tid;
-    bstore_nt(rr, (unsigned)(f0 + (e & 15)) + ((unsigned)(e >> 4) << LOG2F), o[s]);
-  }
+  for (int k2 = 0; k2 < 16; ++k2) bstore_nt(rr, (unsigned)(f0 + g) + ((unsigned)(k1 + 16 * k2) << LOG2F), v[k2]);
 }
 
 // Pass 2 at B = 512 for row y of the batch, f block fblk (16 consecutive f),
