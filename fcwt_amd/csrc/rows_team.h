@@ -851,6 +851,7 @@ struct TeamMulti {
   unsigned* ctl;  // [kSyncAbort] abort code, [kSyncExited] finished work-groups; zeroed before launch
   int used;
 };
+static_assert(sizeof(TeamMulti) <= 4096, "kernel arguments are limited to 4 KiB");
 
 __global__ __launch_bounds__(kTeamThreads, kTeamWgPerCu) void k_rows_team_multi(TeamMulti m) {
   __shared__ int s_flag, s_next;
