@@ -135,77 +135,110 @@ __device__ __forceinline__ void direct_block(const KArgs& a, const Item it, floa
       for (int k1 = 1; k1 < R1; ++k1) v[k1] = cmul(v[k1], cmul(base, wm[k1 * P2 + j]));
     }
     __syncthreads();  // the tables are read: the exchange reuses their space
-    // exchange in two rounds, phases [0, G/2) then [G/2, G): g = tid / P2,
-    // so each round is a whole set of wavefronts
-    constexpr int GH = C::G / 2;
+    if constexpr (LOG2M == 9) {
+      // M = 512 (R1 = 32, P2 = 16, 16 phases, 256 threads): the exchange in
+      // two rounds over the halves of k1, as [k1'][g][j] (stride 17 per g),
+      // every thread a writer and a reader in both; the 16-point DFT over j
+      // then runs on thread (g2, kr) = (tid & 15, tid >> 4), so that lanes
+      // g2 = 0..15 hold 16 consecutive phases: each store writes 128-byte
+      // runs of the row straight from the registers (m = 16 h + kr + 32 k2),
+      // with no output transpose through LDS
+      static_assert(R1 == 32 && P2 == 16 && C::G == 16 && NT == 256, "M = 512 layout");
+      constexpr int KS = C::G * 17;
+      const int g2 = tid & 15, kr = tid >> 4;
+      float2* out = row_out(a, r);
+      const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const bool mine = active && g >= h * GH && g < (h + 1) * GH;
-      const int gl = g - h * GH;
-      if (mine) {
+      for (int h = 0; h < 2; ++h) {
+        if (active) {
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) buf[gl * C::XS + j * (R1 + 1) + k1] = v[k1];
-      }
-      __syncthreads();
-      if (mine) {
-        constexpr int U = R1 / P2;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int k1 = u * P2 + j;
-          float2 w[P2];
-#pragma unroll
-          for (int jj = 0; jj < P2; ++jj) w[jj] = buf[gl * C::XS + jj * (R1 + 1) + k1];
-          dft_reg<P2, 1>(w);
-#pragma unroll
-          for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
+          for (int k = 0; k < 16; ++k) buf[k * KS + g * 17 + j] = v[16 * h + k];
         }
-      }
-      __syncthreads();
-    }
-    // transpose to [m][g] in two rounds over m = k1 + R1 k2 (k2 < P2/2, then
-    // the rest), each followed by its 128-byte output runs
-    float2* out = row_out(a, r);
-    constexpr int MH = M / 2, KH = P2 / 2;
+        __syncthreads();
+        float2 w[16];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      if (active) {
-        constexpr int U = R1 / P2;
+        for (int jj = 0; jj < 16; ++jj) w[jj] = buf[kr * KS + g2 * 17 + jj];
+        dft_reg<16, 1>(w);
+        if (g2 < G) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int k2 = 0; k2 < KH; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + h * KH + k2];
-      }
-      __syncthreads();
-      if (G == C::G) {
-        // every LDS read first, then the stores, clipped at t >= n by the
-        // row's buffer range
-        constexpr int S = MH * C::G / NT;
-        float2 o[S];
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const int e = s * NT + tid;
-          o[s] = buf[(e / C::G) * C::OS + e % C::G];
+          for (int k2 = 0; k2 < 16; ++k2)
+            bstore_nt(rr, (unsigned)(l0 + g2) + ((unsigned)(16 * h + kr + 32 * k2) << log2L), w[k2]);
         }
-        const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const int e = s * NT + tid;
-          // runs shorter than 128 bytes (M = 1024 in the team launch) go
-          // without the non-temporal hint, so that the L2 joins the runs of
-          // neighbouring blocks (1.3% of the 384-channel batch, measured)
-          if constexpr (TEAM && C::G < 16)
-            bstore(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
-          else
-            bstore_nt(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
-        }
-      } else {
-        for (int e = tid; e < MH * G; e += NT) {
-          const int mm = e / G, gg = e - mm * G;
-          const long long t = (long long)(l0 + gg) + ((long long)(mm + h * MH) << log2L);
-          if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[mm * C::OS + gg]);
-        }
+        if (h == 0) __syncthreads();  // round 0's reads are done before round 1's writes
       }
-      if (h == 0) __syncthreads();
+    } else {
+      // exchange in two rounds, phases [0, G/2) then [G/2, G): g = tid / P2,
+      // so each round is a whole set of wavefronts
+      constexpr int GH = C::G / 2;
+  #pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bool mine = active && g >= h * GH && g < (h + 1) * GH;
+        const int gl = g - h * GH;
+        if (mine) {
+  #pragma unroll
+          for (int k1 = 0; k1 < R1; ++k1) buf[gl * C::XS + j * (R1 + 1) + k1] = v[k1];
+        }
+        __syncthreads();
+        if (mine) {
+          constexpr int U = R1 / P2;
+  #pragma unroll
+          for (int u = 0; u < U; ++u) {
+            const int k1 = u * P2 + j;
+            float2 w[P2];
+  #pragma unroll
+            for (int jj = 0; jj < P2; ++jj) w[jj] = buf[gl * C::XS + jj * (R1 + 1) + k1];
+            dft_reg<P2, 1>(w);
+  #pragma unroll
+            for (int k2 = 0; k2 < P2; ++k2) v[u * P2 + k2] = w[k2];
+          }
+        }
+        __syncthreads();
+      }
+      // transpose to [m][g] in two rounds over m = k1 + R1 k2 (k2 < P2/2, then
+      // the rest), each followed by its 128-byte output runs
+      float2* out = row_out(a, r);
+      constexpr int MH = M / 2, KH = P2 / 2;
+  #pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (active) {
+          constexpr int U = R1 / P2;
+  #pragma unroll
+          for (int u = 0; u < U; ++u)
+  #pragma unroll
+            for (int k2 = 0; k2 < KH; ++k2) buf[((u * P2 + j) + R1 * k2) * C::OS + g] = v[u * P2 + h * KH + k2];
+        }
+        __syncthreads();
+        if (G == C::G) {
+          // every LDS read first, then the stores, clipped at t >= n by the
+          // row's buffer range
+          constexpr int S = MH * C::G / NT;
+          float2 o[S];
+  #pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const int e = s * NT + tid;
+            o[s] = buf[(e / C::G) * C::OS + e % C::G];
+          }
+          const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
+  #pragma unroll
+          for (int s = 0; s < S; ++s) {
+            const int e = s * NT + tid;
+            // runs shorter than 128 bytes (M = 1024 in the team launch) go
+            // without the non-temporal hint, so that the L2 joins the runs of
+            // neighbouring blocks (1.3% of the 384-channel batch, measured)
+            if constexpr (TEAM && C::G < 16)
+              bstore(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
+            else
+              bstore_nt(rr, (unsigned)(l0 + e % C::G) + ((unsigned)(e / C::G + h * MH) << log2L), o[s]);
+          }
+        } else {
+          for (int e = tid; e < MH * G; e += NT) {
+            const int mm = e / G, gg = e - mm * G;
+            const long long t = (long long)(l0 + gg) + ((long long)(mm + h * MH) << log2L);
+            if (t < r.n) gstore_nt((gfloat2*)(out + t), buf[mm * C::OS + gg]);
+          }
+        }
+        if (h == 0) __syncthreads();
+      }
     }
   } else {
     // single thread per phase: only the phase factor W_N^{q l} = W_{R1 L}^{k l}
