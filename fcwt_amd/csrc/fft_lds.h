@@ -68,6 +68,16 @@ __device__ __forceinline__ void bstore2_nt(__amdgpu_buffer_rsrc_t r, unsigned id
   __builtin_amdgcn_raw_buffer_store_b128(u, r, idx * 8u, 0, 2);
 }
 
+// The value that lane l ^ 16 holds in x (lanes 16 apart, within each
+// 32-lane half of the wave), from v_permlane16_swap: that instruction swaps
+// the odd 16-lane rows of its first operand with the even rows of its
+// second, so with x in both, the first result holds the partner's value in
+// the odd rows and the second in the even rows.
+__device__ __forceinline__ float lane_xor16(float x, bool odd_row) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(odd_row ? r[0] : r[1]);
+}
+
 constexpr int kLdsCap = 16384;  // complex64 elements per work-group (128 KiB)
 constexpr int kBlock = 1024;    // threads per LDS-FFT work-group
 

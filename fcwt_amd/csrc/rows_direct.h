@@ -166,6 +166,65 @@ __device__ __forceinline__ void direct_block(const KArgs& a, const Item it, floa
         }
         if (h == 0) __syncthreads();  // round 0's reads are done before round 1's writes
       }
+    } else if constexpr (LOG2M == 10) {
+      // M = 1024 (R1 = 32, P2 = 32; 16 phases and 512 threads, or 8 phases
+      // and 256 threads in the team launch): the same two rounds over the
+      // halves of k1, as [k1'][g][j] (stride 33 per g).  The 32-point DFT
+      // over j of one (g, k1) runs on the lane pair e = 0, 1 (lanes 16 apart)
+      // of thread (g2, e, kr), g2 the low lane bits: a 16-point DFT over the
+      // j = 2 j' + e, E_e[m2'], then the radix-2 step through
+      // v_permlane16_swap,
+      //   Y[m2' + 16 e] = E_0[m2'] + (-1)^e W_32^{m2'} E_1[m2'],
+      // lane e writing its own e; m = k1 + 32 (m2' + 16 e), and lanes
+      // g2 = 0 .. G-1 store runs of G phases from the registers (128 bytes,
+      // or 64 in the team launch).  Both layouts do the same arithmetic per
+      // phase, so their outputs are identical.
+      constexpr int GS = C::G;
+      static_assert(R1 == 32 && P2 == 32 && (GS == 16 || (TEAM && GS == 8)) && NT == 32 * GS, "M = 1024 layout");
+      constexpr int JS = 33, KS = GS * JS;
+      static_assert(16 * KS + 16 <= C::LDS, "exchange and W_32 table fit the block's LDS");
+      float2* const w32 = buf + 16 * KS;  // W_32^m, m < 16 (= W_N^{32 m L})
+      if (tid < 16) w32[tid] = a.tw((unsigned)(32 * tid) << log2L);
+      const int g2 = tid & (GS - 1), e = (tid >> 4) & 1;
+      const int kr = GS == 16 ? tid >> 5 : ((tid >> 3) & 1) + 2 * (tid >> 5);
+      const bool odd = e != 0;
+      float2* out = row_out(a, r);
+      const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (active) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) buf[k * KS + g * JS + j] = v[16 * h + k];
+        }
+        __syncthreads();
+        float2 w[16];
+#pragma unroll
+        for (int jp = 0; jp < 16; ++jp) w[jp] = buf[kr * KS + g2 * JS + 2 * jp + e];
+        dft_reg<16, 1>(w);  // E_e[m2']
+#pragma unroll
+        for (int m2 = 0; m2 < 16; ++m2) {
+          float2 x = w[m2];
+          if (m2 > 0) {
+            const float2 wx = cmul(x, w32[m2]);
+            x = odd ? wx : x;
+          }
+          const float2 pr = make_float2(lane_xor16(x.x, odd), lane_xor16(x.y, odd));
+          w[m2] = odd ? csub(pr, x) : cadd(x, pr);
+        }
+        if (g2 < G) {
+#pragma unroll
+          for (int m2 = 0; m2 < 16; ++m2) {
+            const unsigned t = (unsigned)(l0 + g2) + ((unsigned)(16 * h + kr + 32 * (m2 + 16 * e)) << log2L);
+            // 64-byte runs (team launch) without the non-temporal hint, so
+            // that the L2 joins the runs of neighbouring blocks
+            if constexpr (GS < 16)
+              bstore(rr, t, w[m2]);
+            else
+              bstore_nt(rr, t, w[m2]);
+          }
+        }
+        if (h == 0) __syncthreads();  // round 0's reads are done before round 1's writes
+      }
     } else {
       // exchange in two rounds, phases [0, G/2) then [G/2, G): g = tid / P2,
       // so each round is a whole set of wavefronts

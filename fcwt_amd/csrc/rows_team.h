@@ -305,16 +305,6 @@ __shared__ float2 tm_post[1 << kTeamMaxLog2F];  // pass-1 [k1][l'][j]
 __shared__ float2 tm_t2[kTeamB];                // pass-2 [g][k]
 __shared__ float2 tm_wb[512];                   // pass-2 [k1][j] (B = 256) or [m1][j] (B = 512)
 
-// The value that lane l ^ 16 holds in x (lanes 16 apart, within each
-// 32-lane half of the wave), from v_permlane16_swap: that instruction swaps
-// the odd 16-lane rows of its first operand with the even rows of its
-// second, so with x in both, the first result holds the partner's value in
-// the odd rows and the second in the even rows.
-__device__ __forceinline__ float lane_xor16(float x, bool odd_row) {
-  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
-  return __uint_as_float(odd_row ? r[0] : r[1]);
-}
-
 // Pass 1 for one row and the member's Qb = 4096/F qb values, A = 2^LOG2A,
 // B = 2^LOG2B.
 //   band (tm_buf + zs_off)  the row's Z[qb0 + ql + B qf] at [ql * (A + P2) + qf]
