@@ -544,18 +544,37 @@ __global__ __launch_bounds__(kRowFftThreads) void k_adj_rows(const float2* grad,
     }
     __syncthreads();
     row_fft<LOG2H, -1, true>(s);
-    // the row's band: the last bin any of its segments covers
+    // the window of each of this thread's bins: one scan of the row's
+    // segments for all of them (the first segment holding q, as
+    // row_window), then every gather of the mother table issued before the
+    // first is used; bins outside every segment (beyond the band) are
+    // neither windowed nor summed
     const RowD r = rows[i];
-    int qend = 0;
-    for (int k = 0; k < r.seg_count; ++k) qend = max(qend, segs[r.seg_begin + k].hi);
+    int idx[KQ];
+#pragma unroll
+    for (int u = 0; u < KQ; ++u) idx[u] = -1;
+    for (int k = 0; k < r.seg_count; ++k) {
+#pragma clang fp contract(off)
+      const SegD sg = segs[r.seg_begin + k];
+#pragma unroll
+      for (int u = 0; u < KQ; ++u) {
+        const int q = threadIdx.x + u * kRowFftThreads;
+        if (idx[u] < 0 && q < HALF && q >= sg.lo && q < sg.hi) {
+          const float v = sg.step * (float)q;
+          idx[u] = (int)(v < maxidx ? v : maxidx);
+        }
+      }
+    }
+    float wv[KQ];
+#pragma unroll
+    for (int u = 0; u < KQ; ++u) wv[u] = idx[u] >= 0 ? mother[idx[u]] : 0.f;
 #pragma unroll
     for (int u = 0; u < KQ; ++u) {
       const int q = threadIdx.x + u * kRowFftThreads;
-      if (q < HALF && q < qend) {
-        const float w = row_window(segs, r, q, maxidx, mother);
+      if (idx[u] >= 0) {
         const float2 z = s.data[row_pad(q)];
-        acc[u].x += w * z.x;
-        acc[u].y += w * z.y;
+        acc[u].x += wv[u] * z.x;
+        acc[u].y += wv[u] * z.y;
       }
     }
   }
