@@ -2181,6 +2181,9 @@ static int launch_batch(fcwt_hip_batch b, const std::vector<std::vector<const fl
     if (rc != FCWT_HIP_OK) return rc;
     if (defer) {
       const int s = kTeamMaxLog2N - p->log2N;  // seg[0] holds N = 2^18
+      if (s < 0 || s >= kTeamMaxSegs || (m.used >> s & 1))  // team plans are 2^15 .. 2^18, one per size
+        return fail(FCWT_HIP_EUNSUPPORTED, "fcwt_hip_batch_execute_device: no team-launch slot for N = 2^" +
+                                               std::to_string(p->log2N));
       ta.sync = b->d_msync + b->msync_off[k];
       m.seg[s] = ta;
       m.used |= 1 << s;
