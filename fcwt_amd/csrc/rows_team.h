@@ -479,7 +479,9 @@ __device__ __forceinline__ void team_pass2(gfloat2* __restrict__ out, int n, int
 template <int LOG2F>
 __device__ __forceinline__ void team_pass2_b512(gfloat2* __restrict__ out, int n, int f0, const gfloat2* U,
                                                 float2 base, unsigned* done) {
-  constexpr int JS = 33, XS = 16 * JS + 4;  // exchange [g][m1][j]
+  // exchange [g][m1][j]: strides 34 per m1 and 548 per g, so that the 32
+  // lanes of one stage-2 read (8 g, m1 & 1, e) fall on distinct banks
+  constexpr int JS = 34, XS = 16 * JS + 4;
   float2* const xb = tm_buf;
   const float2* const t2 = tm_t2;
   const float2* const wb = tm_wb;

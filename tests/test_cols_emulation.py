@@ -130,7 +130,7 @@ def test_team_pass2_512_point_decomposition():
     U[f][qb] (row stride 512), qb = j + 32 k, m = m1 + 16 m2.  Stage 1 (thread
     g = tid >> 5, j = tid & 31): W_N^{32 k f} from the [g][k] table, a 16-point
     DFT over k, then c W_N^{j f} W_512^{j m1}; the exchange xb[g][m1][j] at
-    g * 532 + m1 * 33 + j; stage 2 (g = tid & 7, e = (tid >> 4) & 1,
+    g * 548 + m1 * 34 + j; stage 2 (g = tid & 7, e = (tid >> 4) & 1,
     m1 = ((tid >> 3) & 1) + 2 (tid >> 5)): a 16-point DFT over j' of
     xb[g][m1][2 j' + e]; lane e = 1 multiplies by W_32^{m2'}, the lanes 16
     apart (tid ^ 16: same g and m1, the other e) swap, e = 0 writes
@@ -150,7 +150,7 @@ def test_team_pass2_512_point_decomposition():
     for f in range(F):
         ref[f + F * np.arange(B)] = np.fft.ifft(U[f] * W(N, np.arange(B) * f)) * B
     wb = W(N, (np.arange(16)[:, None] * np.arange(32)[None, :]) << log2F)  # [m1][j]
-    JS, XS = 33, 16 * 33 + 4
+    JS, XS = 34, 16 * 34 + 4
     got = np.full(N, np.nan, complex)
     k = np.arange(16)
     tids = np.arange(256)
