@@ -177,3 +177,29 @@ def test_one_team_launch_failure_is_reported_by_every_plan(monkeypatch):
     got = _run_batch(b, xs, [sc.size] * len(xs))
     for k in (0, 4, 8):
         assert np.array_equal(got[k], run_device(xs[k], sc, 6.0)), lengths[k]
+
+
+def test_team_launch_beside_column_path_and_graph_replays():
+    """Signals padded to 2^16, 2^18 and 2^20: the first two share one team
+    launch, the 2^20 signal's wide rows take the column path in launches of
+    their own; executes with the same buffers replay the captured graph.
+    Every signal equals its one-signal plan bit for bit, on every replay."""
+    lengths = [40_000, 150_000, 600_000]
+    xs = [tones_noise(n, fs=30_000, seed=n + 5, fmax=12_000) for n in lengths]
+    sc = O.scales(O.LOGSCALES, 30_000, 10, 10_000, 24)
+    b = fcwt_amd.Batch(lengths, sc, 6.0)
+    shp = b.shape()
+    assert shp["plans"] == 3 and shp["row_launches"] > 1, shp
+    refs = [run_device(x, sc, 6.0) for x in xs]
+    st = torch.cuda.current_stream()
+    dx = [torch.from_numpy(x).cuda() for x in xs]
+    outs = [torch.empty((sc.size, x.size), dtype=torch.complex64, device="cuda") for x in xs]
+    for k in range(4):  # eager, capture, two replays
+        for o in outs:
+            o.zero_()
+        b.execute_device([d.data_ptr() for d in dx], [o.data_ptr() for o in outs], st.cuda_stream)
+        torch.cuda.synchronize()
+        b.check()
+        for o, r, n in zip(outs, refs, lengths):
+            assert np.array_equal(o.cpu().numpy(), r), (k, n)
+    assert b.graph_launches() == 3
