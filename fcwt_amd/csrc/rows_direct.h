@@ -50,7 +50,10 @@ struct DirectCfg {
   static constexpr int TABLES = M + M + R1 * G;
   static constexpr int BUFH = (G / 2) * XS > (M / 2) * OS ? (G / 2) * XS : (M / 2) * OS;
   static constexpr int BUF = P2 > 1 ? 0 : M * OS;  // offset of the tables
-  static constexpr int LDS = P2 > 1 ? (BUFH > TABLES ? BUFH : TABLES) : BUF + TABLES;
+  // at M = 256 the exchange holds 16 k1 x G phases x (P2 + 1) values per round
+  static constexpr int EXCH = LOG2M == 8 ? 16 * G * (P2 + 1) : 0;
+  static constexpr int LDS0 = P2 > 1 ? (BUFH > TABLES ? BUFH : TABLES) : BUF + TABLES;
+  static constexpr int LDS = LDS0 > EXCH ? LDS0 : EXCH;
 };
 
 // One block of a row: G phases of item `it`, by C::NT threads (the whole
@@ -163,6 +166,42 @@ __device__ __forceinline__ void direct_block(const KArgs& a, const Item it, floa
 #pragma unroll
           for (int k2 = 0; k2 < 16; ++k2)
             bstore_nt(rr, (unsigned)(l0 + g2) + ((unsigned)(16 * h + kr + 32 * k2) << log2L), w[k2]);
+        }
+        if (h == 0) __syncthreads();  // round 0's reads are done before round 1's writes
+      }
+    } else if constexpr (LOG2M == 8) {
+      // M = 256 (R1 = 32, P2 = 8, 32 phases, 256 threads, in either kind of
+      // launch): the two rounds over the halves of k1, as [k1'][g][j]
+      // (stride 9 per g); thread (g2, gb, kr) = (tid & 15, (tid >> 4) & 1,
+      // tid >> 5) runs the 8-point DFTs over j of phase 16 gb + g2 for
+      // k1' = kr and kr + 8, and its lanes g2 = 0..15 store 128-byte runs
+      // from the registers (m = 16 h + k1' + 32 k2)
+      static_assert(R1 == 32 && P2 == 8 && C::G == 32 && NT == 256, "M = 256 layout");
+      constexpr int GS = P2 + 1, KS = C::G * GS;
+      static_assert(16 * KS <= C::LDS, "exchange fits the block's LDS");
+      const int g2 = tid & 15, gb = (tid >> 4) & 1, kr = tid >> 5;
+      const int gp = 16 * gb + g2;
+      float2* out = row_out(a, r);
+      const __amdgpu_buffer_rsrc_t rr = row_rsrc(out, r.n);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        if (active) {
+#pragma unroll
+          for (int k = 0; k < 16; ++k) buf[k * KS + g * GS + j] = v[16 * h + k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+          const int kk = kr + 8 * d;
+          float2 w[8];
+#pragma unroll
+          for (int jj = 0; jj < 8; ++jj) w[jj] = buf[kk * KS + gp * GS + jj];
+          dft_reg<8, 1>(w);
+          if (gp < G) {
+#pragma unroll
+            for (int k2 = 0; k2 < 8; ++k2)
+              bstore_nt(rr, (unsigned)(l0 + gp) + ((unsigned)(16 * h + kk + 32 * k2) << log2L), w[k2]);
+          }
         }
         if (h == 0) __syncthreads();  // round 0's reads are done before round 1's writes
       }

@@ -1,4 +1,4 @@
-"""CPU emulation of the direct kernel's exchange at M = 512 and M = 1024
+"""CPU emulation of the direct kernel's exchange at M = 256, 512 and 1024
 (fcwt_amd/csrc/rows_direct.h, direct_block): the LDS exchange indices of
 the two rounds over the halves of k1, the thread mapping of the second-stage
 DFTs (g in the low lane bits) and, at M = 1024, the 16-point halves on lanes
@@ -29,7 +29,7 @@ def stage1(Z, M, N, G, l0):
     return v
 
 
-@pytest.mark.parametrize("M,G", [(512, 16), (1024, 16), (1024, 8)])
+@pytest.mark.parametrize("M,G", [(256, 32), (512, 16), (1024, 16), (1024, 8)])
 def test_direct_exchange_and_register_stores(M, G):
     rng = np.random.default_rng(M + G)
     N = 1 << 14
@@ -40,7 +40,27 @@ def test_direct_exchange_and_register_stores(M, G):
     P2 = M // 32
     NT = G * P2
     out = {}
-    if M == 512:
+    if M == 256:
+        GS, KS = 9, G * 9
+        assert 16 * KS <= 4768
+        for h in range(2):
+            buf = np.full(16 * KS, np.nan, complex)
+            for tid in range(NT):  # writers (g, j)
+                g, j = tid // P2, tid % P2
+                for kk in range(16):
+                    buf[kk * KS + g * GS + j] = v[g, j, 16 * h + kk]
+            for tid in range(NT):  # readers (g2, gb, kr): k1' = kr, kr + 8 of phase 16 gb + g2
+                g2, gb, kr = tid & 15, (tid >> 4) & 1, tid >> 5
+                gp = 16 * gb + g2
+                for d in range(2):
+                    kk = kr + 8 * d
+                    w = buf[kk * KS + gp * GS + np.arange(8)]
+                    y = np.array([np.sum(w * W(8, np.arange(8) * k2)) for k2 in range(8)])
+                    for k2 in range(8):
+                        t = l0 + gp + L * (16 * h + kk + 32 * k2)
+                        assert t not in out
+                        out[t] = y[k2]
+    elif M == 512:
         KS = G * 17
         for h in range(2):
             buf = np.full(4352, np.nan, complex)
