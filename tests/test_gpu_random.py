@@ -1,0 +1,99 @@
+"""GPU parity on seeded random plans: signal length (log-uniform in 64 ..
+600,000; the drawn cases pad to 2^9 ... 2^19 and reach the direct, team and
+column paths), sample rate, scale type (src/fcwt/fcwt.cpp:111-190), frequency range, number
+of scales, Morlet sigma, thread count (which sets how the reference's shared
+O1 buffer is reused between scales, fcwt.cpp:449-529), normalisation, the
+scalar-coverage flag and real or complex input (fcwt.cpp:536-538), each
+drawn per case from a fixed seed.  Sampled rows (the first, the last and
+random ones) against the float64 oracle.
+
+Bar: relative L2 <= 1e-5 per row, and rows the oracle leaves all-zero are
+zero.
+"""
+import numpy as np
+import pytest
+
+import fcwt_amd
+from fcwt_amd import _native as nat
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+CASES = 24
+
+
+def _draw(case):
+    rng = np.random.default_rng(9000 + case)
+    n = int(np.exp(rng.uniform(np.log(64), np.log(600_000))))
+    fs = int(rng.integers(100, 50_000))
+    st = int(rng.integers(0, 3))
+    lo, hi = sorted(np.exp(rng.uniform(np.log(1e-4), np.log(0.5), 2)))
+    f0, f1 = lo * fs, max(hi * fs, lo * fs + 0.02)
+    fn = int(rng.integers(1, 300))
+    fb = float(rng.uniform(0.5, 12.0))
+    threads = int(rng.choice([1, 2, 4, 8]))
+    normalize = bool(rng.integers(0, 2))
+    flags = nat.COVERAGE_SCALAR if rng.uniform() < 0.25 else 0
+    complex_input = bool(rng.uniform() < 0.25)
+    t = np.arange(n) / fs
+    fr = rng.uniform(-0.45 if complex_input else 0.0, 0.45, 4) * fs
+    if complex_input:
+        x = np.exp(2j * np.pi * fr[:, None] * t[None, :]).sum(axis=0)
+        x = (x + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    else:
+        x = np.sin(2 * np.pi * fr[:, None] * t[None, :] + rng.uniform(0, 6.3, 4)[:, None]).sum(axis=0)
+        x = (x + 0.3 * rng.standard_normal(n)).astype(np.float32)
+    rows = sorted({0, fn - 1} | set(rng.integers(0, fn, min(fn, 6)).tolist()))
+    return dict(n=n, fs=fs, st=st, f0=f0, f1=f1, fn=fn, fb=fb, threads=threads, normalize=normalize,
+                flags=flags, complex_input=complex_input, x=x, rows=rows)
+
+
+def _describe(c):
+    return ", ".join(f"{k}={c[k]}" for k in ("n", "fs", "st", "f0", "f1", "fn", "fb", "threads", "normalize",
+                                              "flags", "complex_input"))
+
+
+@pytest.mark.parametrize("case", range(CASES))
+def test_random_plan(case):
+    import torch
+
+    c = _draw(case)
+    print(f"case {case}: {_describe(c)}", flush=True)
+    sc = O.scales(c["st"], c["fs"], c["f0"], c["f1"], c["fn"])
+    p = fcwt_amd.Plan(c["n"], sc, c["fb"], c["threads"], c["normalize"], c["flags"])
+    dx = torch.from_numpy(c["x"]).to("cuda:0")
+    out = torch.empty((c["fn"], c["n"]), dtype=torch.complex64, device="cuda:0")
+    p.execute_device(dx.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream,
+                     complex_input=c["complex_input"])
+    torch.cuda.synchronize()
+    p.check()
+    got = out.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    sel = c["rows"]
+    avx = not (c["flags"] & nat.COVERAGE_SCALAR)
+    ref = O.cwt_fp64(c["x"], sc, c["fb"], c["threads"], avx, c["normalize"], rows=sel)
+    nz = np.linalg.norm(ref, axis=1) > 0
+    assert np.all(got[sel][~nz] == 0), "rows the oracle leaves all-zero must be zero"
+    err = O.rel_l2_rows(got[sel], ref)
+    worst = int(np.argmax(np.where(nz, err, 0))) if nz.any() else 0
+    assert err[nz].max(initial=0) <= TOL, f"row {sel[worst]}: rel-L2 {err[worst]:.3e} ({_describe(c)})"
+
+
+def test_random_plans_cover_every_path():
+    """The drawn cases reach the direct, team and column row paths (the
+    column path's rows count as two-pass rows) and both input kinds (plan
+    shapes only, no GPU work)."""
+    kinds, paths = set(), {"direct": 0, "team": 0, "two_pass": 0}
+    sizes = set()
+    for case in range(CASES):
+        c = _draw(case)
+        sc = O.scales(c["st"], c["fs"], c["f0"], c["f1"], c["fn"])
+        split = fcwt_amd.Plan(c["n"], sc, c["fb"], c["threads"], c["normalize"], c["flags"]).row_split()
+        for k, v in split.items():
+            paths[k] += v
+        kinds.add(c["complex_input"])
+        sizes.add(O.find2power(c["n"]))
+    assert kinds == {False, True}
+    assert min(paths.values()) > 0, paths
+    assert max(sizes) >= 19 and min(sizes) <= 10, sorted(sizes)
