@@ -5,7 +5,8 @@ of scales, Morlet sigma, thread count (which sets how the reference's shared
 O1 buffer is reused between scales, fcwt.cpp:449-529), normalisation, the
 scalar-coverage flag and real or complex input (fcwt.cpp:536-538), each
 drawn per case from a fixed seed.  Sampled rows (the first, the last and
-random ones) against the float64 oracle.
+random ones) against the float64 oracle.  Seeded random batches of uneven
+channels (fcwt_hip_batch_*) against one-signal plans, bit for bit.
 
 Bar: relative L2 <= 1e-5 per row, and rows the oracle leaves all-zero are
 zero.
@@ -97,3 +98,65 @@ def test_random_plans_cover_every_path():
     assert kinds == {False, True}
     assert min(paths.values()) > 0, paths
     assert max(sizes) >= 19 and min(sizes) <= 10, sorted(sizes)
+
+
+BATCHES = 6
+
+
+def _draw_batch(case):
+    rng = np.random.default_rng(9500 + case)
+    cnt = int(rng.integers(3, 11))
+    fs = int(rng.integers(100, 50_000))
+    fb = float(rng.uniform(0.5, 12.0))
+    threads = int(rng.choice([1, 2, 4, 8]))
+    normalize = bool(rng.integers(0, 2))
+    flags = nat.COVERAGE_SCALAR if rng.uniform() < 0.25 else 0
+    lengths, xs, scs = [], [], []
+    for _ in range(cnt):
+        n = int(np.exp(rng.uniform(np.log(64), np.log(300_000))))
+        lo, hi = sorted(np.exp(rng.uniform(np.log(1e-4), np.log(0.5), 2)))
+        sc = O.scales(int(rng.integers(0, 3)), fs, lo * fs, max(hi * fs, lo * fs + 0.02), int(rng.integers(1, 120)))
+        t = np.arange(n) / fs
+        fr = rng.uniform(0.0, 0.45, 3) * fs
+        x = np.sin(2 * np.pi * fr[:, None] * t[None, :]).sum(axis=0) + 0.3 * rng.standard_normal(n)
+        lengths.append(n)
+        xs.append(x.astype(np.float32))
+        scs.append(sc)
+    return dict(lengths=lengths, xs=xs, scs=scs, fb=fb, threads=threads, normalize=normalize, flags=flags)
+
+
+@pytest.mark.parametrize("case", range(BATCHES))
+def test_random_batch(case):
+    """A seeded batch of 3 .. 10 channels of random length (64 .. 300,000),
+    each with its own random scales, one sigma, thread count, normalisation
+    and coverage flag for the batch: every channel equals, bit for bit, the
+    one-signal plan of the same arguments, and the first and last row of
+    every channel meet the bar against the float64 oracle."""
+    import torch
+
+    c = _draw_batch(case)
+    print(f"batch {case}: lengths {c['lengths']}, fns {[s.size for s in c['scs']]}, fb={c['fb']}, "
+          f"threads={c['threads']}, normalize={c['normalize']}, flags={c['flags']}", flush=True)
+    args = (c["fb"], c["threads"], c["normalize"], c["flags"])
+    b = fcwt_amd.Batch(c["lengths"], c["scs"], *args)
+    dx = [torch.from_numpy(x).cuda() for x in c["xs"]]
+    outs = [torch.empty((s.size, x.size), dtype=torch.complex64, device="cuda") for s, x in zip(c["scs"], c["xs"])]
+    b.execute_device([d.data_ptr() for d in dx], [o.data_ptr() for o in outs],
+                     torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    b.check()
+    avx = not (c["flags"] & nat.COVERAGE_SCALAR)
+    for k, (x, sc, o) in enumerate(zip(c["xs"], c["scs"], outs)):
+        got = o.cpu().numpy()
+        p = fcwt_amd.Plan(x.size, sc, *args)
+        single = torch.empty_like(o)
+        p.execute_device(dx[k].data_ptr(), single.data_ptr(), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        p.check()
+        assert np.array_equal(got.view(np.uint64), single.cpu().numpy().view(np.uint64)), f"channel {k} (n={x.size})"
+        rows = sorted({0, sc.size - 1})
+        ref = O.cwt_fp64(x, sc, c["fb"], c["threads"], avx, c["normalize"], rows=rows)
+        nz = np.linalg.norm(ref, axis=1) > 0
+        assert np.all(got[rows][~nz] == 0)
+        err = O.rel_l2_rows(got[rows], ref)
+        assert err[nz].max(initial=0) <= TOL, f"channel {k} (n={x.size}): rel-L2 {err.max():.3e}"
