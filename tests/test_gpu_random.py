@@ -6,7 +6,9 @@ O1 buffer is reused between scales, fcwt.cpp:449-529), normalisation, the
 scalar-coverage flag and real or complex input (fcwt.cpp:536-538), each
 drawn per case from a fixed seed.  Sampled rows (the first, the last and
 random ones) against the float64 oracle.  Seeded random batches of uneven
-channels (fcwt_hip_batch_*) against one-signal plans, bit for bit.
+channels (fcwt_hip_batch_*) against one-signal plans, bit for bit, and the
+input gradient (fcwt_hip_adjoint_device) of seeded random plans against the
+float64 oracle adjoint.
 
 Bar: relative L2 <= 1e-5 per row, and rows the oracle leaves all-zero are
 zero.
@@ -160,3 +162,43 @@ def test_random_batch(case):
         assert np.all(got[rows][~nz] == 0)
         err = O.rel_l2_rows(got[rows], ref)
         assert err[nz].max(initial=0) <= TOL, f"channel {k} (n={x.size}): rel-L2 {err.max():.3e}"
+
+
+ADJOINTS = 8
+
+
+@pytest.mark.parametrize("case", range(ADJOINTS))
+def test_random_adjoint(case):
+    """The input gradient (fcwt_hip_adjoint_device) of a seeded random plan
+    (length 64 .. 60,000, up to 120 scales of a random type and range,
+    random sigma, thread count, normalisation, coverage flag and input kind)
+    for a random output gradient, against the float64 oracle adjoint:
+    relative L2 <= 1e-5 of the gradient vector."""
+    import torch
+
+    rng = np.random.default_rng(9800 + case)
+    n = int(np.exp(rng.uniform(np.log(64), np.log(60_000))))
+    fs = int(rng.integers(100, 50_000))
+    lo, hi = sorted(np.exp(rng.uniform(np.log(1e-4), np.log(0.5), 2)))
+    sc = O.scales(int(rng.integers(0, 3)), fs, lo * fs, max(hi * fs, lo * fs + 0.02), int(rng.integers(1, 120)))
+    fb = float(rng.uniform(0.5, 12.0))
+    threads = int(rng.choice([1, 2, 4, 8]))
+    normalize = bool(rng.integers(0, 2))
+    flags = nat.COVERAGE_SCALAR if rng.uniform() < 0.25 else 0
+    complex_input = bool(rng.uniform() < 0.3)
+    print(f"adjoint {case}: n={n}, fn={sc.size}, fb={fb}, threads={threads}, normalize={normalize}, "
+          f"flags={flags}, complex_input={complex_input}", flush=True)
+    G = (rng.standard_normal((sc.size, n)) + 1j * rng.standard_normal((sc.size, n))).astype(np.complex64)
+    p = fcwt_amd.Plan(n, sc, fb, threads, normalize, flags)
+    g = torch.from_numpy(G).cuda()
+    xbar = torch.empty(n, dtype=torch.complex64 if complex_input else torch.float32, device="cuda")
+    p.adjoint_device(g.data_ptr(), xbar.data_ptr(), torch.cuda.current_stream().cuda_stream, complex_input)
+    torch.cuda.synchronize()
+    got = xbar.cpu().numpy().astype(np.complex128)
+    ref = O.adjoint_fp64(G, sc, fb, threads, not (flags & nat.COVERAGE_SCALAR), normalize, complex_input)
+    ref = np.asarray(ref, np.complex128)
+    if not np.linalg.norm(ref):
+        assert not np.any(got)
+        return
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err <= TOL, f"rel-L2 {err:.3e}"
